@@ -1,0 +1,182 @@
+// Device-side helpers shared by the projector / backprojector kernels (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "fc_internal.h"
+
+namespace fc {
+
+// Stored element types. 16-bit storage keeps values as raw bits; arithmetic is always FP32.
+template <int DT>
+struct Store;
+template <>
+struct Store<FC_F32> {
+  using T = float;
+};
+template <>
+struct Store<FC_BF16> {
+  using T = uint16_t;
+};
+template <>
+struct Store<FC_F16> {
+  using T = uint16_t;
+};
+
+__device__ __forceinline__ float bf16_bits_to_f32(uint32_t b) { return __uint_as_float(b << 16); }
+
+// FP32 -> BF16 with round-to-nearest-even; NaN stays NaN.
+__device__ __forceinline__ uint16_t f32_to_bf16_bits(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float f16_bits_to_f32(uint32_t b) {
+  return (float)__builtin_bit_cast(_Float16, (uint16_t)(b & 0xffffu));
+}
+__device__ __forceinline__ uint16_t f32_to_f16_bits(float f) {
+  return __builtin_bit_cast(uint16_t, (_Float16)f);
+}
+
+template <int DT>
+__device__ __forceinline__ float load_elem(const void* p, size_t i) {
+  if constexpr (DT == FC_F32) {
+    return static_cast<const float*>(p)[i];
+  } else if constexpr (DT == FC_BF16) {
+    return bf16_bits_to_f32(static_cast<const uint16_t*>(p)[i]);
+  } else {
+    return f16_bits_to_f32(static_cast<const uint16_t*>(p)[i]);
+  }
+}
+
+template <int DT>
+__device__ __forceinline__ uint16_t to_bits16(float f) {
+  if constexpr (DT == FC_BF16) {
+    return f32_to_bf16_bits(f);
+  } else {
+    return f32_to_f16_bits(f);
+  }
+}
+
+// Unpack the low / high 16-bit halves of a 32-bit word to FP32.
+template <int DT>
+__device__ __forceinline__ float lo16(uint32_t w) {
+  if constexpr (DT == FC_BF16) {
+    return __uint_as_float(w << 16);
+  } else {
+    return f16_bits_to_f32(w);
+  }
+}
+template <int DT>
+__device__ __forceinline__ float hi16(uint32_t w) {
+  if constexpr (DT == FC_BF16) {
+    return __uint_as_float(w & 0xffff0000u);
+  } else {
+    return f16_bits_to_f32(w >> 16);
+  }
+}
+
+template <int DT>
+__device__ __forceinline__ uint32_t pack2(float a, float b) {
+  return (uint32_t)to_bits16<DT>(a) | ((uint32_t)to_bits16<DT>(b) << 16);
+}
+
+__device__ __forceinline__ float lerp(float a, float b, float t) { return fmaf(t, b - a, a); }
+
+// Logical block index for a 1-D grid, remapped so each of the 8 XCDs (which receive blocks
+// round-robin) works on one contiguous range of logical blocks.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+  const int nxcd = 8;
+  int xcd = b % nxcd, idx = b / nxcd;
+  int per = nb / nxcd, rem = nb % nxcd;
+  return xcd * per + (xcd < rem ? xcd : rem) + idx;
+}
+
+// Ray setup for the forward projector and its adjoint. Computes, in FP64, the range of integer k
+// for which s + k*h*u lies inside the closed box [-1, n] of the local block (index space), and
+// returns the first sample and the step in the padded index frame (offset `pad`) as FP32.
+// nk = 0 when the ray misses the block.
+struct RaySeg {
+  float px, py, pz;  // first sample, padded index frame
+  float sx, sy, sz;  // step h*u
+  int nk;
+};
+
+__device__ __forceinline__ void ray_dir(const double* V, int j, int i, double* dx, double* dy,
+                                        double* dz) {
+  if (V[24] == (double)FC_DET_CYLINDER) {
+    double g = ((double)j - V[25]) * V[26];
+    double sg, cg;
+    sincos(g, &sg, &cg);
+    double r = (double)i - V[27];
+    *dx = cg * V[3] + sg * V[6] + r * V[9];
+    *dy = cg * V[4] + sg * V[7] + r * V[10];
+    *dz = cg * V[5] + sg * V[8] + r * V[11];
+  } else {
+    *dx = V[3] + j * V[6] + i * V[9] - V[0];
+    *dy = V[4] + j * V[7] + i * V[10] - V[1];
+    *dz = V[5] + j * V[8] + i * V[11] - V[2];
+  }
+}
+
+__device__ __forceinline__ RaySeg ray_segment(const double* V, int j, int i, double ox,
+                                              double oy, double oz, int nx, int ny, int nz,
+                                              double h, int max_steps, float pad) {
+  RaySeg r;
+  r.nk = 0;
+  r.px = r.py = r.pz = 0.f;
+  r.sx = r.sy = r.sz = 0.f;
+  double dx, dy, dz;
+  ray_dir(V, j, i, &dx, &dy, &dz);
+  double len = sqrt(dx * dx + dy * dy + dz * dz);
+  if (!(len > 0.0)) return r;
+  double inv = 1.0 / len;
+  double ux = dx * inv, uy = dy * inv, uz = dz * inv;
+  double lx = V[0] - ox, ly = V[1] - oy, lz = V[2] - oz;  // source, local index space
+  double tmin = -1e300, tmax = 1e300;
+  const double l[3] = {lx, ly, lz};
+  const double u[3] = {ux, uy, uz};
+  const double n[3] = {(double)nx, (double)ny, (double)nz};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (fabs(u[a]) < 1e-300) {
+      if (l[a] <= -1.0 || l[a] >= n[a]) return r;
+    } else {
+      double ru = 1.0 / u[a];
+      double t1 = (-1.0 - l[a]) * ru, t2 = (n[a] - l[a]) * ru;
+      tmin = fmax(tmin, fmin(t1, t2));
+      tmax = fmin(tmax, fmax(t1, t2));
+    }
+  }
+  if (!(tmin <= tmax)) return r;
+  double k0 = ceil(tmin / h), k1 = floor(tmax / h);
+  if (!(k1 >= k0)) return r;
+  double nkd = k1 - k0 + 1.0;
+  if (!(nkd <= (double)max_steps)) return r;  // only for corrupt geometry
+  double t0 = k0 * h;
+  r.px = (float)(lx + t0 * ux + pad);
+  r.py = (float)(ly + t0 * uy + pad);
+  r.pz = (float)(lz + t0 * uz + pad);
+  r.sx = (float)(h * ux);
+  r.sy = (float)(h * uy);
+  r.sz = (float)(h * uz);
+  int nk = (int)nkd;
+  // Memory-safety guard. Sample k is evaluated as fmaf((float)k, s, p0), which is monotone in k,
+  // so checking the first and last sample bounds every sample. In exact arithmetic they lie in
+  // [pad-1, n+pad]; the guard allows up to (pad-1) voxels of FP32 rounding slack.
+  float kl = (float)(nk - 1);
+  float qx = fmaf(kl, r.sx, r.px), qy = fmaf(kl, r.sy, r.py), qz = fmaf(kl, r.sz, r.pz);
+  float lo = 0.0f;
+  float hx = (float)nx + 2.0f * pad - 1.0f, hy = (float)ny + 2.0f * pad - 1.0f,
+        hz = (float)nz + 2.0f * pad - 1.0f;
+  bool ok = r.px >= lo && r.py >= lo && r.pz >= lo && qx >= lo && qy >= lo && qz >= lo &&
+            r.px < hx && r.py < hy && r.pz < hz && qx < hx && qy < hy && qz < hz;
+  r.nk = ok ? nk : 0;
+  return r;
+}
+
+}  // namespace fc

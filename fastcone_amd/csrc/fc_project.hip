@@ -1,0 +1,409 @@
+// Forward projector (ray marching, trilinear) and its exact transpose, for gfx950.
+//
+// Discretisation (coneBeamGPU.cu:96-176, generalised to step h): for each detector pixel the ray
+// from the source is sampled at integer multiples k*h of the step, and
+//   proj += norm * h * sum_k T(src + k h u),
+// T being the trilinear interpolant of the block extended by zeros. The reference sums hardware
+// texture fetches (low-precision fixed-point weights) over a fixed bounding-sphere sweep with an
+// accumulated position; here each ray is clipped in FP64 to the exact support box [-1, n]^3 and
+// positions are recomputed per sample with one FMA, and the interpolation weights are FP32.
+//
+// Volume staging ("gather layouts", chosen because HBM is plentiful and gathers are the cost):
+//   OCTET: for every base voxel b in [-2, n] the 8 corner values v[b + {0,1}^3] are stored
+//          contiguously (32 B in FP32, 16 B in BF16/FP16), so one trilinear sample is one
+//          (FP32: two adjacent) 16-byte load. 8x the volume's bytes.
+//   PLAIN: the volume with a 2-voxel zero halo; one sample is four dword-aligned 8-byte loads
+//          of x-neighbour pairs. FP32 only.
+// The halo / padding makes every sample's corners in-bounds without per-tap checks.
+#include <hip/hip_runtime.h>
+
+#include <cstdlib>
+#include <cstring>
+
+#include "fc_device.h"
+#include "fc_internal.h"
+
+namespace fc {
+namespace {
+
+constexpr int kTile = 16;  // 16x16 pixel tile per 256-thread block, 8x8 per wave
+
+typedef float f32x2_a4 __attribute__((ext_vector_type(2), aligned(4)));
+
+enum FpLayout { FP_OCTET = 0, FP_PLAIN = 1 };
+
+int fp_layout_choice() {
+  const char* e = std::getenv("FASTCONE_FP_LAYOUT");
+  if (e && std::strcmp(e, "plain") == 0) return FP_PLAIN;
+  return FP_OCTET;
+}
+
+// ---- staging ---------------------------------------------------------------------------------
+
+// Octets over base positions [-2, n] on each axis (padded index b + 2 in [0, n+2]).
+template <int DT_IN>
+__global__ __launch_bounds__(256) void stage_octets(const void* __restrict__ vol, int nx, int ny,
+                                                    int nz, void* __restrict__ oct, int PX,
+                                                    int PY) {
+  int bx = blockIdx.x * 256 + threadIdx.x;
+  int by = blockIdx.y, bz = blockIdx.z;
+  if (bx >= PX) return;
+  int x0 = bx - 2, y0 = by - 2, z0 = bz - 2;
+  float v[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    int x = x0 + (c & 1), y = y0 + ((c >> 1) & 1), z = z0 + (c >> 2);
+    bool in = x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz;
+    v[c] = in ? load_elem<DT_IN>(vol, ((size_t)z * ny + y) * nx + x) : 0.0f;
+  }
+  size_t o = ((size_t)bz * PY + by) * PX + bx;
+  if constexpr (DT_IN == FC_F32) {
+    float4* d = static_cast<float4*>(oct) + 2 * o;
+    d[0] = make_float4(v[0], v[1], v[2], v[3]);
+    d[1] = make_float4(v[4], v[5], v[6], v[7]);
+  } else {
+    uint4* d = static_cast<uint4*>(oct) + o;
+    *d = make_uint4(pack2<DT_IN>(v[0], v[1]), pack2<DT_IN>(v[2], v[3]), pack2<DT_IN>(v[4], v[5]),
+                    pack2<DT_IN>(v[6], v[7]));
+  }
+}
+
+// Plain FP32 copy with a zero halo of 2 (padded dims n + 4).
+template <int DT_IN>
+__global__ __launch_bounds__(256) void stage_plain(const void* __restrict__ vol, int nx, int ny,
+                                                   int nz, float* __restrict__ out, int PX,
+                                                   int PY) {
+  int px = blockIdx.x * 256 + threadIdx.x;
+  int py = blockIdx.y, pz = blockIdx.z;
+  if (px >= PX) return;
+  int x = px - 2, y = py - 2, z = pz - 2;
+  bool in = x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz;
+  out[((size_t)pz * PY + py) * PX + px] =
+      in ? load_elem<DT_IN>(vol, ((size_t)z * ny + y) * nx + x) : 0.0f;
+}
+
+// ---- projector -------------------------------------------------------------------------------
+
+struct FpArgs {
+  const void* data;  // staged volume
+  float* proj;
+  const double* views;
+  const float* norm;
+  double ox, oy, oz;
+  double h;
+  int nx, ny, nz;
+  int PX, PY;  // staged pitch in base positions (octet) or elements (plain)
+  int nw, nh, nviews;
+  int tiles_w, tiles_per_view, group;
+  int max_steps;
+};
+
+// Tile order: groups of `group` consecutive views; inside a group the same pixel tile of all
+// its views is adjacent, so concurrently running blocks of one XCD gather the same voxels.
+__device__ __forceinline__ void fp_tile(const FpArgs& a, int* t, int* j, int* i) {
+  int lb = xcd_remap(blockIdx.x, gridDim.x);
+  int per_group = a.tiles_per_view * a.group;
+  int g = lb / per_group;
+  int r = lb - g * per_group;
+  int gsize = min(a.group, a.nviews - g * a.group);
+  int tile = r / gsize;
+  int v = r - tile * gsize;
+  *t = g * a.group + v;
+  int ty = tile / a.tiles_w, tx = tile - ty * a.tiles_w;
+  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  *j = tx * kTile + (wave & 1) * 8 + (lane & 7);
+  *i = ty * kTile + (wave >> 1) * 8 + (lane >> 3);
+}
+
+template <int DT, int LAYOUT>
+__device__ __forceinline__ float trilinear_sample(const FpArgs& a, float qx, float qy, float qz) {
+  float fx = floorf(qx), fy = floorf(qy), fz = floorf(qz);
+  float ax = qx - fx, ay = qy - fy, az = qz - fz;
+  uint32_t ix = (uint32_t)(int)fx, iy = (uint32_t)(int)fy, iz = (uint32_t)(int)fz;
+  uint32_t o = (iz * (uint32_t)a.PY + iy) * (uint32_t)a.PX + ix;
+  float v0, v1, v2, v3, v4, v5, v6, v7;
+  if constexpr (LAYOUT == FP_OCTET) {
+    if constexpr (DT == FC_F32) {
+      const float4* p = static_cast<const float4*>(a.data) + 2 * (size_t)o;
+      float4 lo = p[0], hi = p[1];
+      v0 = lo.x; v1 = lo.y; v2 = lo.z; v3 = lo.w;
+      v4 = hi.x; v5 = hi.y; v6 = hi.z; v7 = hi.w;
+    } else {
+      uint4 w = static_cast<const uint4*>(a.data)[o];
+      v0 = lo16<DT>(w.x); v1 = hi16<DT>(w.x); v2 = lo16<DT>(w.y); v3 = hi16<DT>(w.y);
+      v4 = lo16<DT>(w.z); v5 = hi16<DT>(w.z); v6 = lo16<DT>(w.w); v7 = hi16<DT>(w.w);
+    }
+  } else {
+    const float* p = static_cast<const float*>(a.data) + o;
+    size_t sy = (size_t)a.PX, sz = (size_t)a.PX * a.PY;
+    f32x2_a4 r0 = *reinterpret_cast<const f32x2_a4*>(p);
+    f32x2_a4 r1 = *reinterpret_cast<const f32x2_a4*>(p + sy);
+    f32x2_a4 r2 = *reinterpret_cast<const f32x2_a4*>(p + sz);
+    f32x2_a4 r3 = *reinterpret_cast<const f32x2_a4*>(p + sz + sy);
+    v0 = r0.x; v1 = r0.y; v2 = r1.x; v3 = r1.y;
+    v4 = r2.x; v5 = r2.y; v6 = r3.x; v7 = r3.y;
+  }
+  float c00 = lerp(v0, v1, ax), c10 = lerp(v2, v3, ax);
+  float c01 = lerp(v4, v5, ax), c11 = lerp(v6, v7, ax);
+  float c0 = lerp(c00, c10, ay), c1 = lerp(c01, c11, ay);
+  return lerp(c0, c1, az);
+}
+
+template <int DT, int LAYOUT>
+__global__ __launch_bounds__(256) void fp_kernel(FpArgs a) {
+  int t, j, i;
+  fp_tile(a, &t, &j, &i);
+  if (t >= a.nviews || j >= a.nw || i >= a.nh) return;
+  const double* V = a.views + (size_t)t * FC_VIEW_DOUBLES;
+  RaySeg r = ray_segment(V, j, i, a.ox, a.oy, a.oz, a.nx, a.ny, a.nz, a.h, a.max_steps, 2.0f);
+  float acc = 0.0f;
+  for (int k = 0; k < r.nk; ++k) {
+    float kf = (float)k;
+    acc += trilinear_sample<DT, LAYOUT>(a, fmaf(kf, r.sx, r.px), fmaf(kf, r.sy, r.py),
+                                        fmaf(kf, r.sz, r.pz));
+  }
+  float scale = (a.norm ? a.norm[t] : 1.0f) * (float)a.h;
+  size_t idx = ((size_t)t * a.nh + i) * a.nw + j;
+  a.proj[idx] += scale * acc;
+}
+
+// ---- exact transpose (scatter) ---------------------------------------------------------------
+
+struct AdjArgs {
+  float* acc;  // padded FP32 accumulator, halo 2, dims (n+4)
+  const void* proj;
+  const double* views;
+  const float* norm;
+  double ox, oy, oz;
+  double h;
+  int nx, ny, nz;
+  int PX, PY;
+  int nw, nh, nviews;
+  int tiles_w, tiles_per_view, group;
+  int max_steps;
+};
+
+template <int DT>
+__global__ __launch_bounds__(256) void adjoint_kernel(AdjArgs a) {
+  FpArgs tmp;
+  tmp.tiles_w = a.tiles_w;
+  tmp.tiles_per_view = a.tiles_per_view;
+  tmp.group = a.group;
+  tmp.nviews = a.nviews;
+  int t, j, i;
+  fp_tile(tmp, &t, &j, &i);
+  if (t >= a.nviews || j >= a.nw || i >= a.nh) return;
+  size_t pidx = ((size_t)t * a.nh + i) * a.nw + j;
+  float y = load_elem<DT>(a.proj, pidx) * (a.norm ? a.norm[t] : 1.0f) * (float)a.h;
+  if (y == 0.0f) return;
+  const double* V = a.views + (size_t)t * FC_VIEW_DOUBLES;
+  RaySeg r = ray_segment(V, j, i, a.ox, a.oy, a.oz, a.nx, a.ny, a.nz, a.h, a.max_steps, 2.0f);
+  const size_t sy = (size_t)a.PX, sz = (size_t)a.PX * a.PY;
+  for (int k = 0; k < r.nk; ++k) {
+    float kf = (float)k;
+    float qx = fmaf(kf, r.sx, r.px), qy = fmaf(kf, r.sy, r.py), qz = fmaf(kf, r.sz, r.pz);
+    float fx = floorf(qx), fy = floorf(qy), fz = floorf(qz);
+    float ax = qx - fx, ay = qy - fy, az = qz - fz;
+    uint32_t o = ((uint32_t)(int)fz * (uint32_t)a.PY + (uint32_t)(int)fy) * (uint32_t)a.PX +
+                 (uint32_t)(int)fx;
+    float* p = a.acc + o;
+    float w0z = y * (1.0f - az), w1z = y * az;
+    float w00 = w0z * (1.0f - ay), w10 = w0z * ay, w01 = w1z * (1.0f - ay), w11 = w1z * ay;
+    unsafeAtomicAdd(p, w00 * (1.0f - ax));
+    unsafeAtomicAdd(p + 1, w00 * ax);
+    unsafeAtomicAdd(p + sy, w10 * (1.0f - ax));
+    unsafeAtomicAdd(p + sy + 1, w10 * ax);
+    unsafeAtomicAdd(p + sz, w01 * (1.0f - ax));
+    unsafeAtomicAdd(p + sz + 1, w01 * ax);
+    unsafeAtomicAdd(p + sz + sy, w11 * (1.0f - ax));
+    unsafeAtomicAdd(p + sz + sy + 1, w11 * ax);
+  }
+}
+
+// vol[z][y][x] += acc[z+2][y+2][x+2]
+__global__ __launch_bounds__(256) void unpad_add(const float* __restrict__ acc, int PX, int PY,
+                                                 float* __restrict__ vol, int nx, int ny, int nz) {
+  int x = blockIdx.x * 256 + threadIdx.x;
+  int y = blockIdx.y, z = blockIdx.z;
+  if (x >= nx) return;
+  vol[((size_t)z * ny + y) * nx + x] += acc[((size_t)(z + 2) * PY + (y + 2)) * PX + (x + 2)];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void zero_fill(T* __restrict__ p, size_t n) {
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * 256;
+  for (; i < n; i += stride) p[i] = T(0);
+}
+
+// ---- sample count ----------------------------------------------------------------------------
+
+struct CountArgs {
+  const double* views;
+  long long* counts;
+  double ox, oy, oz;
+  double h;
+  int nx, ny, nz;
+  int nw, nh, nviews;
+  int tiles_w, tiles_per_view;
+  int max_steps;
+};
+
+__global__ __launch_bounds__(256) void count_kernel(CountArgs a) {
+  __shared__ int part[4];
+  int tile = blockIdx.x, t = blockIdx.y;
+  int ty = tile / a.tiles_w, tx = tile - ty * a.tiles_w;
+  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int j = tx * kTile + (wave & 1) * 8 + (lane & 7);
+  int i = ty * kTile + (wave >> 1) * 8 + (lane >> 3);
+  int nk = 0;
+  if (j < a.nw && i < a.nh) {
+    const double* V = a.views + (size_t)t * FC_VIEW_DOUBLES;
+    nk = ray_segment(V, j, i, a.ox, a.oy, a.oz, a.nx, a.ny, a.nz, a.h, a.max_steps, 2.0f).nk;
+  }
+  // wave reduction
+  for (int off = 32; off > 0; off >>= 1) nk += __shfl_down(nk, off, 64);
+  if (lane == 0) part[wave] = nk;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long s = (long long)part[0] + part[1] + part[2] + part[3];
+    atomicAdd((unsigned long long*)&a.counts[t], (unsigned long long)s);
+  }
+}
+
+int max_steps_for(const int vs[3], double h) {
+  double diag = sqrt((double)(vs[0] + 2) * (vs[0] + 2) + (double)(vs[1] + 2) * (vs[1] + 2) +
+                     (double)(vs[2] + 2) * (vs[2] + 2));
+  double m = ceil(diag / h) + 2.0;
+  return m > 2e9 ? 2000000000 : (int)m;
+}
+
+}  // namespace
+
+int launch_project(int device, const void* vol, int dt, const int vs[3], const double origin[3],
+                   float* proj, const int ps[3], const double* views, double h,
+                   const float* norm, hipStream_t stream) {
+  const int nx = vs[0], ny = vs[1], nz = vs[2];
+  const int layout = (dt == FC_F32) ? fp_layout_choice() : FP_OCTET;
+  int PX, PY, PZ;
+  size_t bytes;
+  if (layout == FP_OCTET) {
+    PX = nx + 3; PY = ny + 3; PZ = nz + 3;
+    bytes = (size_t)PX * PY * PZ * 8 * dtype_size(dt);
+  } else {
+    PX = nx + 4; PY = ny + 4; PZ = nz + 4;
+    bytes = (size_t)PX * PY * PZ * 4;
+  }
+  FC_CHECK_ARG(PY <= 65535 && PZ <= 65535, "volume dimension too large for staging grid");
+  FC_CHECK_ARG((size_t)PX * PY * PZ < (1ull << 32), "volume too large for 32-bit voxel offsets");
+  void* ws = nullptr;
+  int rc = acquire_ws(device, SLOT_OCTET, bytes, stream, &ws);
+  if (rc) return rc;
+  dim3 sg((PX + 255) / 256, PY, PZ);
+  if (layout == FP_OCTET) {
+    if (dt == FC_F32) stage_octets<FC_F32><<<sg, 256, 0, stream>>>(vol, nx, ny, nz, ws, PX, PY);
+    else if (dt == FC_BF16) stage_octets<FC_BF16><<<sg, 256, 0, stream>>>(vol, nx, ny, nz, ws, PX, PY);
+    else stage_octets<FC_F16><<<sg, 256, 0, stream>>>(vol, nx, ny, nz, ws, PX, PY);
+  } else {
+    stage_plain<FC_F32><<<sg, 256, 0, stream>>>(vol, nx, ny, nz, (float*)ws, PX, PY);
+  }
+  FC_HIP(hipGetLastError());
+  FpArgs a;
+  a.data = ws;
+  a.proj = proj;
+  a.views = views;
+  a.norm = norm;
+  a.ox = origin[0]; a.oy = origin[1]; a.oz = origin[2];
+  a.h = h;
+  a.nx = nx; a.ny = ny; a.nz = nz;
+  a.PX = PX; a.PY = PY;
+  a.nw = ps[0]; a.nh = ps[1]; a.nviews = ps[2];
+  a.tiles_w = (a.nw + kTile - 1) / kTile;
+  a.tiles_per_view = a.tiles_w * ((a.nh + kTile - 1) / kTile);
+  const char* g = std::getenv("FASTCONE_FP_GROUP");
+  a.group = g ? std::max(1, std::atoi(g)) : 8;
+  a.group = std::min(a.group, std::max(1, a.nviews));
+  a.max_steps = max_steps_for(vs, h);
+  long long nblocks = (long long)a.tiles_per_view * a.nviews;
+  FC_CHECK_ARG(nblocks < (1ll << 31), "too many projection tiles");
+  if (nblocks > 0) {
+    if (layout == FP_OCTET) {
+      if (dt == FC_F32) fp_kernel<FC_F32, FP_OCTET><<<(int)nblocks, 256, 0, stream>>>(a);
+      else if (dt == FC_BF16) fp_kernel<FC_BF16, FP_OCTET><<<(int)nblocks, 256, 0, stream>>>(a);
+      else fp_kernel<FC_F16, FP_OCTET><<<(int)nblocks, 256, 0, stream>>>(a);
+    } else {
+      fp_kernel<FC_F32, FP_PLAIN><<<(int)nblocks, 256, 0, stream>>>(a);
+    }
+    FC_HIP(hipGetLastError());
+  }
+  return release_ws(device, SLOT_OCTET, stream);
+}
+
+int launch_adjoint(int device, float* vol, const int vs[3], const double origin[3],
+                   const void* proj, int dt, const int ps[3], const double* views, double h,
+                   const float* norm, hipStream_t stream) {
+  const int nx = vs[0], ny = vs[1], nz = vs[2];
+  const int PX = nx + 4, PY = ny + 4, PZ = nz + 4;
+  FC_CHECK_ARG(PY <= 65535 && PZ <= 65535, "volume dimension too large");
+  FC_CHECK_ARG((size_t)PX * PY * PZ < (1ull << 32), "volume too large for 32-bit voxel offsets");
+  size_t n = (size_t)PX * PY * PZ;
+  void* ws = nullptr;
+  int rc = acquire_ws(device, SLOT_OCTET, n * sizeof(float), stream, &ws);
+  if (rc) return rc;
+  float* acc = static_cast<float*>(ws);
+  zero_fill<<<(unsigned)std::min<size_t>((n + 255) / 256, 65536), 256, 0, stream>>>(acc, n);
+  FC_HIP(hipGetLastError());
+  AdjArgs a;
+  a.acc = acc;
+  a.proj = proj;
+  a.views = views;
+  a.norm = norm;
+  a.ox = origin[0]; a.oy = origin[1]; a.oz = origin[2];
+  a.h = h;
+  a.nx = nx; a.ny = ny; a.nz = nz;
+  a.PX = PX; a.PY = PY;
+  a.nw = ps[0]; a.nh = ps[1]; a.nviews = ps[2];
+  a.tiles_w = (a.nw + kTile - 1) / kTile;
+  a.tiles_per_view = a.tiles_w * ((a.nh + kTile - 1) / kTile);
+  a.group = std::min(8, std::max(1, a.nviews));
+  a.max_steps = max_steps_for(vs, h);
+  long long nblocks = (long long)a.tiles_per_view * a.nviews;
+  FC_CHECK_ARG(nblocks < (1ll << 31), "too many projection tiles");
+  if (nblocks > 0) {
+    if (dt == FC_F32) adjoint_kernel<FC_F32><<<(int)nblocks, 256, 0, stream>>>(a);
+    else if (dt == FC_BF16) adjoint_kernel<FC_BF16><<<(int)nblocks, 256, 0, stream>>>(a);
+    else adjoint_kernel<FC_F16><<<(int)nblocks, 256, 0, stream>>>(a);
+    FC_HIP(hipGetLastError());
+  }
+  dim3 ug((nx + 255) / 256, ny, nz);
+  unpad_add<<<ug, 256, 0, stream>>>(acc, PX, PY, vol, nx, ny, nz);
+  FC_HIP(hipGetLastError());
+  return release_ws(device, SLOT_OCTET, stream);
+}
+
+int launch_sample_count(int device, const int vs[3], const double origin[3], const int ps[3],
+                        const double* views, double h, long long* counts, hipStream_t stream) {
+  (void)device;
+  CountArgs a;
+  a.views = views;
+  a.counts = counts;
+  a.ox = origin[0]; a.oy = origin[1]; a.oz = origin[2];
+  a.h = h;
+  a.nx = vs[0]; a.ny = vs[1]; a.nz = vs[2];
+  a.nw = ps[0]; a.nh = ps[1]; a.nviews = ps[2];
+  a.tiles_w = (a.nw + kTile - 1) / kTile;
+  a.tiles_per_view = a.tiles_w * ((a.nh + kTile - 1) / kTile);
+  a.max_steps = max_steps_for(vs, h);
+  FC_CHECK_ARG(a.nviews <= 65535, "too many views for the count grid");
+  if (a.nviews > 0) {
+    zero_fill<long long><<<(a.nviews + 255) / 256, 256, 0, stream>>>(counts, (size_t)a.nviews);
+    FC_HIP(hipGetLastError());
+  }
+  if (a.nviews > 0 && a.tiles_per_view > 0) {
+    count_kernel<<<dim3(a.tiles_per_view, a.nviews), 256, 0, stream>>>(a);
+    FC_HIP(hipGetLastError());
+  }
+  return FC_OK;
+}
+
+}  // namespace fc

@@ -1,0 +1,172 @@
+/* fastcone.h — C-ABI of libFastcone.so, the MI355X (gfx950) cone-beam projector/backprojector.
+ *
+ * Two groups of entry points:
+ *
+ *  1. The reference ABI, unchanged (same names, argument order and meaning, void return,
+ *     outputs accumulated into caller memory). Each one replaces the function of the same name
+ *     in the reference's wrapper.c. Host pointers; the library copies to and from the GPU.
+ *     Dimension arrays are in C order [x,y,z] / [w,h,t] (wrapper.c:129).
+ *
+ *  2. Additive fc_* entry points (no reference counterpart): device pointers, an explicit HIP
+ *     stream, FP64 per-view geometry records, a step size, storage dtypes and a thread-local error
+ *     code. The Python package (fastcone_amd) and the torch autograd operators call these.
+ *
+ * Errors: the reference checks nothing and exits the process on a bad device
+ * (coneBeamGPU.cu:472-475). Here every entry point validates its arguments and records failures
+ * in a thread-local error code (fc_last_error); the void reference entry points then return
+ * without touching caller memory.
+ */
+#ifndef FASTCONE_H
+#define FASTCONE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------------------------ */
+/* 1. Reference ABI                                                                            */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Backprojection with kernel mode: 0 unweighted general geometry, 1 Katsevich weight U,
+ * otherwise FDK weight U^2. Replaces wrapper.c:112-146 (GPU_CB_K_backproject). */
+void GPU_CB_K_backproject(int GPUID, float* volume, float* projections, float* angles,
+                          float* hMidPlane, float sourceSampleDistance, float pixelWidth,
+                          int volDim[3], int volStart[3], int volEnd[3], int projDim[3],
+                          int projWHSize[2], int* projWOffset, int* projHOffset,
+                          float* normalizationFactor, int kernelMode);
+
+/* Unweighted backprojection onto a misaligned detector (kernel mode 0).
+ * Replaces wrapper.c:149-174 (GPU_CB_misalign_backproject). */
+void GPU_CB_misalign_backproject(int GPUID, float* volume, float* projections, float* angles,
+                                 float* hMidPlane, float sourceSampleDistance, float pixelWidth,
+                                 int volDim[3], int volStart[3], int volEnd[3], int projDim[3],
+                                 int projWHSize[2], int* projWOffset, int* projHOffset,
+                                 float* normalizationFactor, float* misAlignments);
+
+/* FDK backprojection (kernel mode 2). Replaces wrapper.c:177-194 (GPU_CB_backproject). */
+void GPU_CB_backproject(int GPUID, float* volume, float* projections, float* angles,
+                        float* hMidPlane, float sourceSampleDistance, float pixelWidth,
+                        int volDim[3], int volStart[3], int volEnd[3], int projDim[3],
+                        int projWHSize[2], int* projWOffset, int* projHOffset,
+                        float* normalizationFactor);
+
+/* Forward projection with misalignment [Dw, Dh, Dl (pixels), Dphi, Dtheta, Dpsi (radians)],
+ * unit ray step. Replaces wrapper.c:203-227 (GPU_CB_misalign_project). */
+void GPU_CB_misalign_project(int GPUID, float* volume, float* projections, float* angles,
+                             float* hMidPlane, float sourceSampleDistance, float pixelWidth,
+                             int volDim[3], int volStart[3], int volEnd[3], int projDim[3],
+                             int projWHSize[2], int* projWOffset, int* projHOffset,
+                             float* normalizationFactor, float* misAlignments);
+
+/* Forward projection, aligned detector. Replaces wrapper.c:229-249 (GPU_CB_project). */
+void GPU_CB_project(int GPUID, float* volume, float* projections, float* angles,
+                    float* hMidPlane, float sourceSampleDistance, float pixelWidth,
+                    int volDim[3], int volStart[3], int volEnd[3], int projDim[3],
+                    int projWHSize[2], int* projWOffset, int* projHOffset,
+                    float* normalizationFactor);
+
+/* ------------------------------------------------------------------------------------------ */
+/* 2. Additive API                                                                             */
+/* ------------------------------------------------------------------------------------------ */
+
+enum fc_status {
+  FC_OK = 0,
+  FC_ERR_ARGUMENT = 1, /* invalid argument (shape, window, dtype, mode, geometry) */
+  FC_ERR_DEVICE = 2,   /* bad device id or no device */
+  FC_ERR_HIP = 3,      /* a HIP runtime call failed */
+  FC_ERR_MEMORY = 4    /* device allocation failed */
+};
+
+enum fc_dtype { FC_F32 = 0, FC_BF16 = 1, FC_F16 = 2 };
+
+enum fc_bp_mode { FC_BP_UNWEIGHTED = 0, FC_BP_KATSEVICH = 1, FC_BP_FDK = 2 };
+
+enum fc_detector { FC_DET_FLAT = 0, FC_DET_CYLINDER = 1 };
+
+/* Per-view geometry record: FC_VIEW_DOUBLES doubles per view, FP64, voxel coordinates
+ * (a voxel with global index g along an axis of N voxels has coordinate g - N/2).
+ *   [0..2]   source position
+ *   [3..5]   flat: position of window pixel (0,0);  cylinder: SDD * (unit source->axis vector)
+ *   [6..8]   flat: step per window column;          cylinder: SDD * (unit tangential vector)
+ *   [9..11]  step per window row (flat: detector h step; cylinder: row pitch * z)
+ *   [12..23] 3x4 row-major projective matrix M: (x,y,z,1) -> (a,b,c); flat detector: window
+ *            column a/c, row b/c, and c = (distance along the detector normal from the source) /
+ *            (distance of the detector centre), so 1/c is the FDK magnification U.
+ *   [24]     detector kind (fc_detector)
+ *   [25]     cylinder: column of gamma = 0 (window index);  [26] cylinder: column pitch (radians)
+ *   [27]     cylinder: row of v = 0 (window index);         [28] cylinder: SDD
+ *   [29..31] reserved (0)
+ */
+#define FC_VIEW_DOUBLES 32
+
+/* Thread-local status of the calling thread's last fc_* or GPU_CB_* call (fc_status). */
+int fc_last_error(void);
+const char* fc_last_error_message(void);
+void fc_clear_error(void);
+const char* fc_version(void);
+int fc_device_count(void);
+
+/* Fill nviews * FC_VIEW_DOUBLES records for the reference's circular/helical flat-detector
+ * geometry (host memory). Follows coneBeamGPU.cu:1672-1734 (FP64 here, FP32 there):
+ *   angles (radians), hmid (source/detector-centre z index from volume index 0), D source-to-axis
+ *   distance (voxels), pw_w / pw_h detector pitch at the axis (voxels), mis[6] (may be NULL),
+ *   vol_dim_z the whole-volume z size, proj_dim [Nw, Nh] whole-detector size, woff/hoff global
+ *   index of each window's pixel (0,0) (may be NULL = 0). */
+int fc_views_flat(int nviews, const double* angles, const double* hmid, double D, double pw_w,
+                  double pw_h, const double* mis, int vol_dim_z, const int proj_dim[2],
+                  const int* woff, const int* hoff, double* views_out);
+
+/* Cylindrical (curved) detector centred on the source: column j sits at fan angle
+ * (j + woff - Nw/2) * dgamma, row i at height (i + hoff - Nh/2) * dv_row measured at the detector
+ * (voxels). src_z (voxels, coordinate) gives the helical source height per view; R source-to-axis,
+ * SDD source-to-detector (voxels). */
+int fc_views_cylinder(int nviews, const double* angles, const double* src_z, double R, double SDD,
+                      double dgamma, double dv_row, const int proj_dim[2], const int* woff,
+                      const int* hoff, double* views_out);
+
+/* Forward projection on the device:
+ *   proj[t][i][j] += norm[t] * step * sum_k T(src_t + k*step*u_tij)
+ * vol: device, dtype fc_dtype, [nz][ny][nx] (x fastest), voxel (0,0,0) at coordinate origin[];
+ * proj: device float32 [nviews][nh][nw]; views: DEVICE pointer to nviews records; norm: device
+ * float32 [nviews] or NULL (= 1); stream: hipStream_t or NULL. */
+int fc_project_dev(int device, const void* vol, int vol_dtype, const int vol_shape[3],
+                   const double origin[3], float* proj, const int proj_shape[3],
+                   const double* views, double step, const float* norm, void* stream);
+
+/* Exact transpose of fc_project_dev (matched adjoint): vol += A^T proj. vol float32, proj dtype
+ * fc_dtype. Float atomics: summation order is not deterministic. */
+int fc_adjoint_dev(int device, float* vol, const int vol_shape[3], const double origin[3],
+                   const void* proj, int proj_dtype, const int proj_shape[3], const double* views,
+                   double step, const float* norm, void* stream);
+
+/* Voxel-driven backprojection on the device (mode fc_bp_mode; flat detector):
+ *   vol(x) += sum_t norm[t] * w_t(x) * B_t(a/c, b/c),  w = 1 (mode 0), 1/c (mode 1), 1/c^2 (2)
+ * B_t = zero-extended bilinear interpolant of window t. vol float32 [nz][ny][nx]; proj dtype
+ * fc_dtype [nviews][nh][nw]; norm device float32 or NULL (= 1). */
+int fc_backproject_dev(int device, float* vol, const int vol_shape[3], const double origin[3],
+                       const void* proj, int proj_dtype, const int proj_shape[3],
+                       const double* views, const float* norm, int mode, void* stream);
+
+/* Exact sample count of fc_project_dev per view (int64 device array counts[nviews], overwritten). */
+int fc_sample_count_dev(int device, const int vol_shape[3], const double origin[3],
+                        const int proj_shape[3], const double* views, double step,
+                        long long* counts, void* stream);
+
+/* Copy/convert n elements between storage dtypes on the device (f32 <-> bf16/f16,
+ * round-to-nearest-even). */
+int fc_convert_dev(int device, const void* src, int src_dtype, void* dst, int dst_dtype,
+                   long long n, void* stream);
+
+/* Release the library's cached device workspaces on a device. */
+int fc_release_workspace(int device);
+
+/* Out-of-core tiling limits for the reference ABI entry points (host-memory calls): at most
+ * max_voxels voxels of the volume block (split into z slabs) and max_views views per device pass.
+ * 0 = no limit (default: on a 288 GB device the configurations fit whole). Process-wide. */
+int fc_set_tile_limits(long long max_voxels, int max_views);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FASTCONE_H */

@@ -1,0 +1,301 @@
+"""GPU parity: the HIP kernels (through the C-ABI) against the FP64 oracle on the same inputs.
+
+Tolerances (relative L2 = |gpu - oracle| / |oracle|):
+  * FP32 storage: 5e-5 on N(0,1) volumes (FP32 positions and interpolation weights on rough
+    data), 1e-4 on the configuration-0 phantom (the workload target);
+  * BF16 / FP16 storage: the oracle is fed the same rounded input, so the bound is again the FP32
+    arithmetic one (5e-5); the error against the unrounded input is reported.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from fastcone_amd import _native
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _setup(rng, n=12, vol=(18, 20, 22), vol_dim=(24, 24, 26), vol_start=(2, 3, 1),
+           det=(26, 30), det_dim=(30, 34), D=60.0, pw=0.9, mis=None):
+    """C-order vol=(nz,ny,nx), vol_dim=(Nz,Ny,Nx), vol_start=(z,y,x), det=(nh,nw)."""
+    from fastcone_amd.geometry import Geometry
+    ang = np.sort(rng.uniform(0, 2 * np.pi, n))
+    hmid = rng.uniform(vol_dim[0] / 2 - 3, vol_dim[0] / 2 + 3, n)
+    woff = rng.integers(-3, 5, n)
+    hoff = rng.integers(-3, 5, n)
+    g = Geometry.circular(ang, D, pw, vol, det, hmid=hmid, vol_dim=vol_dim, vol_start=vol_start,
+                          woff=woff, hoff=hoff, misalign=mis, det_dim=det_dim)
+    prob = oracle.Problem(ang, hmid, D, pw, vol_dim=vol_dim[::-1], vol_start=vol_start[::-1],
+                          proj_dim=det_dim[::-1], woff=woff, hoff=hoff, mis=mis)
+    return g, prob
+
+
+def _round(x, dtype):
+    return torch.from_numpy(np.asarray(x, np.float32)).to(dtype).float().numpy().astype(np.float64)
+
+
+@pytest.mark.parametrize("h", [1.0, 0.5])
+@pytest.mark.parametrize("mis", [None, [0.6, -0.8, 1.5, 0.03, -0.04, 0.05]])
+def test_project_f32_parity(gpu, h, mis):
+    from fastcone_amd import ops
+    rng = np.random.default_rng(11)
+    g, prob = _setup(rng, mis=mis)
+    x = rng.standard_normal(g.vol_shape).astype(np.float32)
+    norm = rng.uniform(0.5, 1.5, g.nviews).astype(np.float32)
+    p = ops.project(torch.from_numpy(x).to(gpu), g, step=h, norm=torch.from_numpy(norm)).cpu()
+    ref = oracle.project(prob, x, window=g.det_shape[::-1], h=h, norm=norm)
+    assert rel(p, ref) < 5e-5
+
+
+@pytest.mark.parametrize("layout", ["octet", "plain"])
+def test_project_layouts_agree(gpu, layout, monkeypatch):
+    from fastcone_amd import ops
+    monkeypatch.setenv("FASTCONE_FP_LAYOUT", layout)
+    rng = np.random.default_rng(12)
+    g, prob = _setup(rng)
+    x = rng.standard_normal(g.vol_shape).astype(np.float32)
+    p = ops.project(torch.from_numpy(x).to(gpu), g, step=0.5).cpu()
+    ref = oracle.project(prob, x, window=g.det_shape[::-1], h=0.5)
+    assert rel(p, ref) < 5e-5
+
+
+@pytest.mark.parametrize("dtype", ["bfloat16", "float16"])
+def test_project_16bit_storage(gpu, dtype):
+    from fastcone_amd import ops
+    dt = getattr(torch, dtype)
+    rng = np.random.default_rng(13)
+    g, prob = _setup(rng)
+    x = rng.standard_normal(g.vol_shape).astype(np.float32)
+    xs = torch.from_numpy(x).to(gpu).to(dt)
+    p = ops.project(xs, g, step=0.5).cpu().numpy()
+    ref_rounded = oracle.project(prob, _round(x, dt), window=g.det_shape[::-1], h=0.5)
+    ref = oracle.project(prob, x, window=g.det_shape[::-1], h=0.5)
+    assert rel(p, ref_rounded) < 5e-5
+    print(f"{dtype} storage: rel L2 vs FP64 oracle of unrounded input = {rel(p, ref):.3e}")
+    assert rel(p, ref) < (1e-2 if dtype == "bfloat16" else 2e-3)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_backproject_f32_parity(gpu, mode):
+    from fastcone_amd import ops
+    rng = np.random.default_rng(20 + mode)
+    g, prob = _setup(rng)
+    y = rng.standard_normal(g.proj_shape).astype(np.float32)
+    norm = rng.uniform(0.5, 1.5, g.nviews).astype(np.float32)
+    v = ops.backproject(torch.from_numpy(y).to(gpu), g, mode=mode,
+                        norm=torch.from_numpy(norm)).cpu()
+    ref = oracle.backproject(prob, y, g.vol_shape, mode=mode, norm=norm)
+    assert rel(v, ref) < 5e-5
+
+
+def test_backproject_misaligned_mode0(gpu):
+    from fastcone_amd import ops
+    rng = np.random.default_rng(24)
+    g, prob = _setup(rng, mis=[0.6, -0.8, 1.5, 0.03, -0.04, 0.05])
+    y = rng.standard_normal(g.proj_shape).astype(np.float32)
+    v = ops.backproject(torch.from_numpy(y).to(gpu), g, mode=0).cpu()
+    ref = oracle.backproject(prob, y, g.vol_shape, mode=0, norm=np.ones(g.nviews))
+    assert rel(v, ref) < 5e-5
+
+
+@pytest.mark.parametrize("dtype", ["bfloat16", "float16"])
+def test_backproject_16bit_storage(gpu, dtype):
+    from fastcone_amd import ops
+    dt = getattr(torch, dtype)
+    rng = np.random.default_rng(25)
+    g, prob = _setup(rng)
+    y = rng.standard_normal(g.proj_shape).astype(np.float32)
+    v = ops.backproject(torch.from_numpy(y).to(gpu).to(dt), g, mode=2).cpu().numpy()
+    ref_rounded = oracle.backproject(prob, _round(y, dt), g.vol_shape, mode=2,
+                                     norm=np.ones(g.nviews))
+    assert rel(v, ref_rounded) < 5e-5
+
+
+def test_backproject_accumulates_into_out(gpu):
+    from fastcone_amd import ops
+    rng = np.random.default_rng(26)
+    g, prob = _setup(rng)
+    y = rng.standard_normal(g.proj_shape).astype(np.float32)
+    base = rng.standard_normal(g.vol_shape).astype(np.float32)
+    out = torch.from_numpy(base.copy()).to(gpu)
+    ops.backproject(torch.from_numpy(y).to(gpu), g, mode=2, out=out)
+    ref = oracle.backproject(prob, y, g.vol_shape, mode=2, norm=np.ones(g.nviews))
+    assert rel(out.cpu().numpy() - base, ref) < 5e-5
+
+
+@pytest.mark.parametrize("h", [1.0, 0.5])
+def test_adjoint_parity_and_dot_product(gpu, h):
+    from fastcone_amd import ops
+    rng = np.random.default_rng(30)
+    g, prob = _setup(rng, mis=[0.2, 0.1, -0.4, 0.01, 0.02, -0.01])
+    x = rng.standard_normal(g.vol_shape).astype(np.float32)
+    y = rng.standard_normal(g.proj_shape).astype(np.float32)
+    yt = torch.from_numpy(y).to(gpu)
+    aty = ops.adjoint(yt, g, step=h).cpu().numpy()
+    ref = oracle.adjoint(prob, y, g.vol_shape, h=h)
+    assert rel(aty, ref) < 5e-5
+    ax = ops.project(torch.from_numpy(x).to(gpu), g, step=h).cpu().numpy()
+    lhs = np.vdot(ax.astype(np.float64), y.astype(np.float64))
+    rhs = np.vdot(x.astype(np.float64), aty.astype(np.float64))
+    assert abs(lhs - rhs) / abs(lhs) < 1e-5
+
+
+def test_sample_count_exact(gpu):
+    from fastcone_amd import ops
+    rng = np.random.default_rng(31)
+    g, prob = _setup(rng)
+    c = ops.sample_count(g, 0.5, gpu).cpu().numpy()
+    ref = oracle.sample_count(prob, g.vol_shape, window=g.det_shape[::-1], h=0.5)
+    assert np.array_equal(c, ref)
+
+
+def test_convert_round_to_nearest_even(gpu):
+    from fastcone_amd import ops
+    x = torch.randn(100000, device=gpu) * 100
+    for dt in (torch.bfloat16, torch.float16):
+        assert torch.equal(ops.convert(x, dt), x.to(dt))
+        assert torch.equal(ops.convert(x.to(dt), torch.float32), x.to(dt).float())
+
+
+# ---- the reference ABI (host memory) -------------------------------------------------------------
+
+def _abi_case(rng, nv=10):
+    Nz, Ny, Nx = 30, 28, 26
+    vs = np.array([4, 2, 3])          # (z, y, x)
+    shape = (20, 22, 19)
+    vol = rng.standard_normal(shape).astype(np.float32)
+    Nh, Nw = 36, 40
+    win = (30, 33)                    # (h, w)
+    ang = np.sort(rng.uniform(0, 2 * np.pi, nv)).astype(np.float32)
+    z0 = rng.uniform(12, 18, nv).astype(np.float32)
+    woff = rng.integers(-2, 6, nv).astype(np.int32)
+    hoff = rng.integers(-2, 6, nv).astype(np.int32)
+    prob = oracle.Problem(ang.astype(np.float64), z0.astype(np.float64), 70.0, 0.8,
+                          vol_dim=(Nx, Ny, Nz), vol_start=vs[::-1], proj_dim=(Nw, Nh),
+                          woff=woff, hoff=hoff)
+    return dict(vol=vol, vs=vs, dims=np.array([Nz, Ny, Nx]), win=win, ang=ang, z0=z0,
+                woff=woff, hoff=hoff, pdims=np.array([nv, Nh, Nw]), prob=prob)
+
+
+def test_abi_project_matches_oracle_with_reference_conventions(gpu):
+    from fastcone_amd import fastcone as cone
+    rng = np.random.default_rng(40)
+    c = _abi_case(rng)
+    base = rng.standard_normal((len(c["ang"]),) + c["win"]).astype(np.float32)
+    proj = base.copy()
+    cone.project(c["vol"], proj, c["ang"], 70.0, c["z0"], volStart=c["vs"], volDims=c["dims"],
+                 projWOffset=c["woff"], projHOffset=c["hoff"], projDims=c["pdims"],
+                 pixelWidth=0.8)
+    ref = oracle.project(c["prob"], c["vol"], window=c["win"][::-1], h=1.0) + base
+    for t in range(len(c["ang"])):   # the reference zeroes up to global column 0
+        if c["woff"][t] <= 0:
+            ref[t, :, :1 + c["woff"][t]] = 0.0
+    assert rel(proj, ref) < 5e-5
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_abi_backproject_matches_oracle(gpu, mode):
+    from fastcone_amd import fastcone as cone
+    rng = np.random.default_rng(41 + mode)
+    c = _abi_case(rng)
+    y = rng.standard_normal((len(c["ang"]),) + c["win"]).astype(np.float32)
+    base = rng.standard_normal(c["vol"].shape).astype(np.float32)
+    vol = base.copy()
+    cone.backproject(vol, y, c["ang"], 70.0, c["z0"], volStart=c["vs"], volDims=c["dims"],
+                     projWOffset=c["woff"], projHOffset=c["hoff"], projDims=c["pdims"],
+                     kernelMode=mode, pixelWidth=0.8)
+    ref = oracle.backproject(c["prob"], y, vol.shape, mode=mode,
+                             norm=np.full(len(c["ang"]), 1.0 / len(c["ang"])))
+    assert rel(vol - base, ref) < 5e-5
+
+
+def test_abi_misaligned_backproject(gpu):
+    from fastcone_amd import fastcone as cone
+    rng = np.random.default_rng(45)
+    c = _abi_case(rng)
+    mis = np.array([0.6, -0.8, 1.5, 0.03, -0.04, 0.05], np.float32)
+    c["prob"] = oracle.Problem(c["ang"].astype(np.float64), c["z0"].astype(np.float64), 70.0, 0.8,
+                               vol_dim=c["dims"][::-1], vol_start=c["vs"][::-1],
+                               proj_dim=c["pdims"][1:][::-1], woff=c["woff"], hoff=c["hoff"],
+                               mis=mis.astype(np.float64))
+    y = rng.standard_normal((len(c["ang"]),) + c["win"]).astype(np.float32)
+    vol = np.zeros_like(c["vol"])
+    cone.backproject(vol, y, c["ang"], 70.0, c["z0"], volStart=c["vs"], volDims=c["dims"],
+                     projWOffset=c["woff"], projHOffset=c["hoff"], projDims=c["pdims"],
+                     kernelMode=0, misAlignments=mis, pixelWidth=0.8)
+    ref = oracle.backproject(c["prob"], y, vol.shape, mode=0,
+                             norm=np.full(len(c["ang"]), 1.0 / len(c["ang"])))
+    assert rel(vol, ref) < 5e-5
+
+
+def test_abi_tiling_invariance(gpu):
+    from fastcone_amd import fastcone as cone
+    rng = np.random.default_rng(46)
+    c = _abi_case(rng)
+    kw = dict(volStart=c["vs"], volDims=c["dims"], projWOffset=c["woff"], projHOffset=c["hoff"],
+              projDims=c["pdims"], pixelWidth=0.8)
+    p_whole = np.zeros((len(c["ang"]),) + c["win"], np.float32)
+    cone.project(c["vol"], p_whole, c["ang"], 70.0, c["z0"], **kw)
+    v_whole = np.zeros_like(c["vol"])
+    cone.backproject(v_whole, p_whole, c["ang"], 70.0, c["z0"], **kw)
+    try:
+        cone.set_tile_limits(max_voxels=22 * 19 * 3, max_views=3)
+        p_tiled = np.zeros_like(p_whole)
+        cone.project(c["vol"], p_tiled, c["ang"], 70.0, c["z0"], **kw)
+        v_tiled = np.zeros_like(c["vol"])
+        cone.backproject(v_tiled, p_whole, c["ang"], 70.0, c["z0"], **kw)
+    finally:
+        cone.set_tile_limits(0, 0)
+    assert rel(p_tiled, p_whole) < 1e-6
+    assert rel(v_tiled, v_whole) < 1e-6
+
+
+def test_abi_bad_device_raises(gpu):
+    from fastcone_amd import fastcone as cone
+    rng = np.random.default_rng(47)
+    c = _abi_case(rng, nv=2)
+    proj = np.zeros((2,) + c["win"], np.float32)
+    with pytest.raises(_native.FastconeError) as e:
+        cone.project(c["vol"], proj, c["ang"], 70.0, c["z0"], GPUID=999)
+    assert e.value.code == _native.FC_ERR_DEVICE
+    assert np.all(proj == 0)
+
+
+# ---- configuration 0 at full size ----------------------------------------------------------------
+
+def test_config0_full_size_parity(gpu):
+    """128^3 phantom, 180 views over 360 deg, 192^2 detector, pitch 0.75 voxel at the axis,
+    D = 500 voxels, step 0.5: FP and FDK backprojection each within 1e-4 of the FP64 oracle."""
+    from fastcone_amd import ops, phantom
+    from fastcone_amd.geometry import Geometry
+    vol, _ = phantom.config_phantom(0, device=gpu)
+    ang = 2 * np.pi * np.arange(180) / 180
+    g = Geometry.circular(ang, 500.0, 0.75, (128, 128, 128), (192, 192))
+    prob = oracle.Problem(ang, np.full(180, 64.0), 500.0, 0.75, vol_dim=(128, 128, 128),
+                          proj_dim=(192, 192))
+    p = ops.project(vol, g, step=0.5)
+    vol_np = vol.cpu().numpy()
+    ref = oracle.project(prob, vol_np, h=0.5)
+    e_fp = rel(p.cpu(), ref)
+    q = ops.fdk_filter(p, g, 500.0, 0.75)
+    v = ops.backproject(q, g, mode=2, norm=torch.full((180,), 1 / 180, device=gpu)).cpu()
+    ref_bp = oracle.backproject(prob, q.cpu().numpy(), (128, 128, 128), mode=2)
+    e_bp = rel(v, ref_bp)
+    print(f"config 0: FP rel L2 {e_fp:.2e}, FDK BP rel L2 {e_bp:.2e}")
+    assert e_fp < 1e-4 and e_bp < 1e-4
+    # FDK reconstructs the phantom (inside the field of view) to the usual FDK accuracy
+    inner = (slice(40, 88),) * 3
+    err = np.linalg.norm(v.numpy()[inner] - vol_np[inner]) / np.linalg.norm(vol_np[inner])
+    print(f"config 0: FDK vs phantom (central 48^3) rel L2 {err:.3f}")
+    assert err < 0.35
