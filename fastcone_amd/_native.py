@@ -51,6 +51,10 @@ def _declare(lib):
         "fc_convert_dev": (i, [i, _vp, i, _vp, i, ll, _vp]),
         "fc_release_workspace": (i, [i]),
         "fc_set_tile_limits": (i, [ll, i]),
+        "fc_profile_enable": (i, [i]),
+        "fc_gather_bench": (i, [i, i, i, ctypes.POINTER(d)]),
+        "fc_profile_reset": (i, []),
+        "fc_profile_read": (i, [ctypes.c_char_p, ctypes.POINTER(ll), ctypes.POINTER(d)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -80,6 +84,29 @@ def check(rc=None):
     if code != FC_OK:
         msg = L.fc_last_error_message().decode(errors="replace")
         raise FastconeError(code, msg)
+
+
+def profile_enable(on=True):
+    check(lib().fc_profile_enable(1 if on else 0))
+
+
+def profile_reset():
+    check(lib().fc_profile_reset())
+
+
+def profile_read(kernel):
+    """(launch count, total ms) of a kernel since the last reset."""
+    c = ctypes.c_longlong(0)
+    ms = ctypes.c_double(0.0)
+    check(lib().fc_profile_read(kernel.encode(), ctypes.byref(c), ctypes.byref(ms)))
+    return c.value, ms.value
+
+
+def gather_bench(which, dtype=FC_F32, device=0):
+    """Samples/s of the gather microbenchmark (see fc_gather_bench)."""
+    r = ctypes.c_double(0.0)
+    check(lib().fc_gather_bench(int(device), int(which), int(dtype), ctypes.byref(r)))
+    return r.value
 
 
 def header_functions():

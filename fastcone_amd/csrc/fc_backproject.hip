@@ -176,6 +176,29 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
   }
 }
 
+// Gather-roofline microbenchmark for the backprojector: bilinear quad gathers at pseudo-random
+// positions inside one B x B pixel patch per block (same fetch + interpolation code).
+template <int DT>
+__global__ __launch_bounds__(256) void gather_bench_bp(const void* quad, int QW, int QH, int B,
+                                                       int nsamp, float* __restrict__ sink) {
+  uint32_t s = 0x9e3779b9u * (blockIdx.x * 256 + threadIdx.x + 1);
+  int npx = (QW - B - 1) / B, npy = (QH - B - 1) / B;
+  int b = blockIdx.x % (npx * npy);
+  float bu = (float)((b % npx) * B), bv = (float)((b / npx) * B);
+  const float scale = (float)(B - 1) * (1.0f / 16777216.0f);
+  float acc = 0.0f;
+  for (int k = 0; k < nsamp; ++k) {
+    s = s * 1664525u + 1013904223u;
+    float u = bu + (float)(s >> 8) * scale;
+    s = s * 1664525u + 1013904223u;
+    float v = bv + (float)(s >> 8) * scale;
+    float fu = floorf(u), fv = floorf(v);
+    uint32_t idx = (uint32_t)(int)fv * (uint32_t)QW + (uint32_t)(int)fu;
+    acc = fmaf(1.0001f, bilinear_quad<DT>(quad, idx, u - fu, v - fv), acc);
+  }
+  if (acc == 12345.678f) sink[0] = acc;
+}
+
 template <int DT>
 int dispatch_bp(const BpArgs& a, int mode, dim3 grid, hipStream_t s) {
   if (mode == FC_BP_UNWEIGHTED) bp_kernel<DT, FC_BP_UNWEIGHTED><<<grid, 256, 0, s>>>(a);
@@ -216,12 +239,50 @@ int launch_backproject(int device, float* vol, const int vs[3], const double ori
     dim3 grid((vs[0] + kBXY - 1) / kBXY, (vs[1] + kBXY - 1) / kBXY, (vs[2] + kZR - 1) / kZR);
     FC_CHECK_ARG(grid.y <= 65535 && grid.z <= 65535, "volume too large for the grid");
     int r2;
+    int tok = prof_begin("backproject", stream);
     if (dt == FC_F32) r2 = dispatch_bp<FC_F32>(a, mode, grid, stream);
     else if (dt == FC_BF16) r2 = dispatch_bp<FC_BF16>(a, mode, grid, stream);
     else r2 = dispatch_bp<FC_F16>(a, mode, grid, stream);
+    prof_end(tok, stream);
     if (r2) return r2;
   }
   return release_ws(device, SLOT_QUAD, stream);
+}
+
+int launch_gather_bench_bp(int device, int dt, double* samples_per_s) {
+  (void)device;
+  const int QW = 1024, QH = 1024;
+  size_t esz = (dt == FC_F32) ? 16 : 8;
+  void* quad = nullptr;
+  float* sink = nullptr;
+  FC_HIP(hipMalloc(&quad, (size_t)QW * QH * esz));
+  FC_HIP(hipMalloc(&sink, sizeof(float)));
+  FC_HIP(hipMemset(quad, 0x3c, (size_t)QW * QH * esz));
+  hipEvent_t e0, e1;
+  FC_HIP(hipEventCreate(&e0));
+  FC_HIP(hipEventCreate(&e1));
+  const int blocks = 256 * 16, nsamp = 4096;
+  double best = 0.0;
+  for (int B : {8, 16, 32}) {
+    for (int rep = 0; rep < 2; ++rep) {
+      FC_HIP(hipEventRecord(e0, nullptr));
+      if (dt == FC_F32) gather_bench_bp<FC_F32><<<blocks, 256>>>(quad, QW, QH, B, nsamp, sink);
+      else if (dt == FC_BF16) gather_bench_bp<FC_BF16><<<blocks, 256>>>(quad, QW, QH, B, nsamp, sink);
+      else gather_bench_bp<FC_F16><<<blocks, 256>>>(quad, QW, QH, B, nsamp, sink);
+      FC_HIP(hipEventRecord(e1, nullptr));
+      FC_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      FC_HIP(hipEventElapsedTime(&ms, e0, e1));
+      double rate = (double)blocks * 256.0 * nsamp / (ms * 1e-3);
+      if (rep == 1 && rate > best) best = rate;
+    }
+  }
+  FC_HIP(hipEventDestroy(e0));
+  FC_HIP(hipEventDestroy(e1));
+  FC_HIP(hipFree(quad));
+  FC_HIP(hipFree(sink));
+  *samples_per_s = best;
+  return FC_OK;
 }
 
 }  // namespace fc

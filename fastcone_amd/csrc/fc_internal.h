@@ -60,6 +60,12 @@ int acquire_ws(int device, Slot slot, size_t bytes, hipStream_t stream, void** o
 int release_ws(int device, Slot slot, hipStream_t stream);
 int release_all_ws(int device);
 
+// ---- kernel timing (fc_profile_*) ---------------------------------------------------------
+// prof_begin returns a token (>= 0) when profiling is enabled, else -1; prof_end records the stop
+// event. Both are no-ops for token -1.
+int prof_begin(const char* kernel, hipStream_t stream);
+void prof_end(int token, hipStream_t stream);
+
 // ---- dtypes ---------------------------------------------------------------------------------
 inline size_t dtype_size(int dt) { return dt == FC_F32 ? 4 : 2; }
 inline bool dtype_ok(int dt) { return dt == FC_F32 || dt == FC_BF16 || dt == FC_F16; }
@@ -77,6 +83,8 @@ int launch_backproject(int device, float* vol, const int vol_shape[3], const dou
 int launch_sample_count(int device, const int vol_shape[3], const double origin[3],
                         const int proj_shape[3], const double* views, double step,
                         long long* counts, hipStream_t stream);
+int launch_gather_bench_fp(int device, int dt, int plain, double* samples_per_s);
+int launch_gather_bench_bp(int device, int dt, double* samples_per_s);
 int launch_convert(const void* src, int src_dtype, void* dst, int dst_dtype, long long n,
                    hipStream_t stream);
 

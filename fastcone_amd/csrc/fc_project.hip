@@ -19,6 +19,7 @@
 
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 
 #include "fc_device.h"
 #include "fc_internal.h"
@@ -271,6 +272,32 @@ __global__ __launch_bounds__(256) void count_kernel(CountArgs a) {
   }
 }
 
+// ---- gather-roofline microbenchmark -------------------------------------------------------------
+// Each block draws trilinear samples at pseudo-random positions inside one B^3 brick of a staged
+// volume (the brick moves with the block index), with the projector's own fetch + interpolation
+// code, so the sustained rate is the ceiling for that layout's gathers served from L1/L2.
+template <int DT, int LAYOUT>
+__global__ __launch_bounds__(256) void gather_bench_fp(FpArgs a, int B, int nsamp,
+                                                       float* __restrict__ sink) {
+  uint32_t s = 0x9e3779b9u * (blockIdx.x * 256 + threadIdx.x + 1);
+  int nbx = (a.nx - B) / B, nby = (a.ny - B) / B, nbz = (a.nz - B) / B;
+  int b = blockIdx.x % (nbx * nby * nbz);
+  float bx = 2.0f + (float)((b % nbx) * B), by = 2.0f + (float)(((b / nbx) % nby) * B),
+        bz = 2.0f + (float)((b / (nbx * nby)) * B);
+  const float scale = (float)(B - 1) * (1.0f / 16777216.0f);
+  float acc = 0.0f;
+  for (int k = 0; k < nsamp; ++k) {
+    s = s * 1664525u + 1013904223u;
+    float qx = bx + (float)(s >> 8) * scale;
+    s = s * 1664525u + 1013904223u;
+    float qy = by + (float)(s >> 8) * scale;
+    s = s * 1664525u + 1013904223u;
+    float qz = bz + (float)(s >> 8) * scale;
+    acc += trilinear_sample<DT, LAYOUT>(a, qx, qy, qz);
+  }
+  if (acc == 12345.678f) sink[0] = acc;  // keep the work alive
+}
+
 int max_steps_for(const int vs[3], double h) {
   double diag = sqrt((double)(vs[0] + 2) * (vs[0] + 2) + (double)(vs[1] + 2) * (vs[1] + 2) +
                      (double)(vs[2] + 2) * (vs[2] + 2));
@@ -327,6 +354,7 @@ int launch_project(int device, const void* vol, int dt, const int vs[3], const d
   long long nblocks = (long long)a.tiles_per_view * a.nviews;
   FC_CHECK_ARG(nblocks < (1ll << 31), "too many projection tiles");
   if (nblocks > 0) {
+    int tok = prof_begin("project", stream);
     if (layout == FP_OCTET) {
       if (dt == FC_F32) fp_kernel<FC_F32, FP_OCTET><<<(int)nblocks, 256, 0, stream>>>(a);
       else if (dt == FC_BF16) fp_kernel<FC_BF16, FP_OCTET><<<(int)nblocks, 256, 0, stream>>>(a);
@@ -334,6 +362,7 @@ int launch_project(int device, const void* vol, int dt, const int vs[3], const d
     } else {
       fp_kernel<FC_F32, FP_PLAIN><<<(int)nblocks, 256, 0, stream>>>(a);
     }
+    prof_end(tok, stream);
     FC_HIP(hipGetLastError());
   }
   return release_ws(device, SLOT_OCTET, stream);
@@ -370,15 +399,85 @@ int launch_adjoint(int device, float* vol, const int vs[3], const double origin[
   long long nblocks = (long long)a.tiles_per_view * a.nviews;
   FC_CHECK_ARG(nblocks < (1ll << 31), "too many projection tiles");
   if (nblocks > 0) {
+    int tok = prof_begin("adjoint", stream);
     if (dt == FC_F32) adjoint_kernel<FC_F32><<<(int)nblocks, 256, 0, stream>>>(a);
     else if (dt == FC_BF16) adjoint_kernel<FC_BF16><<<(int)nblocks, 256, 0, stream>>>(a);
     else adjoint_kernel<FC_F16><<<(int)nblocks, 256, 0, stream>>>(a);
+    prof_end(tok, stream);
     FC_HIP(hipGetLastError());
   }
   dim3 ug((nx + 255) / 256, ny, nz);
   unpad_add<<<ug, 256, 0, stream>>>(acc, PX, PY, vol, nx, ny, nz);
   FC_HIP(hipGetLastError());
   return release_ws(device, SLOT_OCTET, stream);
+}
+
+// Samples per second of the FP gather (octet layout for `dt`; plain FP32 when plain != 0) on a
+// synthetic n^3 volume, best over brick sizes 4, 8 and 16.
+int launch_gather_bench_fp(int device, int dt, int plain, double* samples_per_s) {
+  const int n = 256;
+  const int vs[3] = {n, n, n};
+  std::vector<float> host((size_t)n * n * n);
+  for (size_t q = 0; q < host.size(); ++q) host[q] = (float)((q * 2654435761u) % 1000) * 1e-3f;
+  float* dvol = nullptr;
+  float* sink = nullptr;
+  FC_HIP(hipMalloc(&dvol, host.size() * sizeof(float)));
+  FC_HIP(hipMalloc(&sink, sizeof(float)));
+  FC_HIP(hipMemcpy(dvol, host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice));
+  void* src = dvol;
+  void* conv = nullptr;
+  if (dt != FC_F32) {
+    FC_HIP(hipMalloc(&conv, host.size() * 2));
+    int rc = launch_convert(dvol, FC_F32, conv, dt, (long long)host.size(), nullptr);
+    if (rc) return rc;
+    src = conv;
+  }
+  const int layout = plain ? FP_PLAIN : FP_OCTET;
+  int PX = n + (plain ? 4 : 3), PY = PX, PZ = PX;
+  size_t bytes = plain ? (size_t)PX * PY * PZ * 4 : (size_t)PX * PY * PZ * 8 * dtype_size(dt);
+  void* ws = nullptr;
+  FC_HIP(hipMalloc(&ws, bytes));
+  dim3 sg((PX + 255) / 256, PY, PZ);
+  if (plain) stage_plain<FC_F32><<<sg, 256>>>(src, n, n, n, (float*)ws, PX, PY);
+  else if (dt == FC_F32) stage_octets<FC_F32><<<sg, 256>>>(src, n, n, n, ws, PX, PY);
+  else if (dt == FC_BF16) stage_octets<FC_BF16><<<sg, 256>>>(src, n, n, n, ws, PX, PY);
+  else stage_octets<FC_F16><<<sg, 256>>>(src, n, n, n, ws, PX, PY);
+  FC_HIP(hipGetLastError());
+  FpArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.data = ws;
+  a.nx = a.ny = a.nz = n;
+  a.PX = PX;
+  a.PY = PY;
+  (void)vs;
+  hipEvent_t e0, e1;
+  FC_HIP(hipEventCreate(&e0));
+  FC_HIP(hipEventCreate(&e1));
+  const int blocks = 256 * 16, nsamp = 2048;
+  double best = 0.0;
+  for (int B : {4, 8, 16}) {
+    for (int rep = 0; rep < 2; ++rep) {
+      FC_HIP(hipEventRecord(e0, nullptr));
+      if (layout == FP_PLAIN) gather_bench_fp<FC_F32, FP_PLAIN><<<blocks, 256>>>(a, B, nsamp, sink);
+      else if (dt == FC_F32) gather_bench_fp<FC_F32, FP_OCTET><<<blocks, 256>>>(a, B, nsamp, sink);
+      else if (dt == FC_BF16) gather_bench_fp<FC_BF16, FP_OCTET><<<blocks, 256>>>(a, B, nsamp, sink);
+      else gather_bench_fp<FC_F16, FP_OCTET><<<blocks, 256>>>(a, B, nsamp, sink);
+      FC_HIP(hipEventRecord(e1, nullptr));
+      FC_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      FC_HIP(hipEventElapsedTime(&ms, e0, e1));
+      double rate = (double)blocks * 256.0 * nsamp / (ms * 1e-3);
+      if (rep == 1 && rate > best) best = rate;
+    }
+  }
+  FC_HIP(hipEventDestroy(e0));
+  FC_HIP(hipEventDestroy(e1));
+  FC_HIP(hipFree(ws));
+  FC_HIP(hipFree(dvol));
+  FC_HIP(hipFree(sink));
+  if (conv) FC_HIP(hipFree(conv));
+  *samples_per_s = best;
+  return FC_OK;
 }
 
 int launch_sample_count(int device, const int vs[3], const double origin[3], const int ps[3],

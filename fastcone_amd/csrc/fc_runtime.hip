@@ -17,6 +17,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <string>
+#include <utility>
 #include <vector>
 
 #include "fc_internal.h"
@@ -119,6 +121,66 @@ int release_all_ws(int device) {
   return FC_OK;
 }
 
+// ---- kernel timing -------------------------------------------------------------------------------
+namespace {
+struct ProfEntry {
+  std::string name;
+  hipEvent_t start = nullptr, stop = nullptr;
+};
+std::mutex g_prof_mutex;
+bool g_prof_on = false;
+std::vector<ProfEntry> g_prof_pending;
+std::vector<std::pair<std::string, std::pair<long long, double>>> g_prof_totals;
+
+void prof_accumulate(const std::string& name, double ms) {
+  for (auto& kv : g_prof_totals) {
+    if (kv.first == name) {
+      kv.second.first += 1;
+      kv.second.second += ms;
+      return;
+    }
+  }
+  g_prof_totals.push_back({name, {1, ms}});
+}
+
+// Resolve every pending entry (waits for its stop event). Caller holds g_prof_mutex.
+int prof_drain() {
+  for (auto& e : g_prof_pending) {
+    FC_HIP(hipEventSynchronize(e.stop));
+    float ms = 0.f;
+    FC_HIP(hipEventElapsedTime(&ms, e.start, e.stop));
+    prof_accumulate(e.name, (double)ms);
+    FC_HIP(hipEventDestroy(e.start));
+    FC_HIP(hipEventDestroy(e.stop));
+  }
+  g_prof_pending.clear();
+  return FC_OK;
+}
+}  // namespace
+
+int prof_begin(const char* kernel, hipStream_t stream) {
+  std::lock_guard<std::mutex> lock(g_prof_mutex);
+  if (!g_prof_on) return -1;
+  ProfEntry e;
+  e.name = kernel;
+  if (hipEventCreate(&e.start) != hipSuccess || hipEventCreate(&e.stop) != hipSuccess) {
+    (void)hipGetLastError();
+    return -1;
+  }
+  if (hipEventRecord(e.start, stream) != hipSuccess) {
+    (void)hipGetLastError();
+    return -1;
+  }
+  g_prof_pending.push_back(e);
+  return (int)g_prof_pending.size() - 1;
+}
+
+void prof_end(int token, hipStream_t stream) {
+  if (token < 0) return;
+  std::lock_guard<std::mutex> lock(g_prof_mutex);
+  if (token < (int)g_prof_pending.size()) (void)hipEventRecord(g_prof_pending[token].stop, stream);
+}
+
 namespace {
 
 int check_shapes(const int vs[3], const int ps[3]) {
@@ -154,6 +216,50 @@ int fc_device_count(void) {
     return 0;
   }
   return n;
+}
+
+int fc_gather_bench(int device, int which, int dtype, double* samples_per_s) {
+  clear_error();
+  FC_CHECK_ARG(samples_per_s, "null output");
+  FC_CHECK_ARG(which >= 0 && which <= 2, "unknown gather benchmark %d", which);
+  FC_CHECK_ARG(dtype_ok(dtype), "unknown dtype %d", dtype);
+  FC_CHECK_ARG(which != 1 || dtype == FC_F32, "the plain layout is FP32 only");
+  int rc = select_device(device);
+  if (rc) return rc;
+  if (which == 2) return launch_gather_bench_bp(device, dtype, samples_per_s);
+  return launch_gather_bench_fp(device, dtype, which == 1, samples_per_s);
+}
+
+int fc_profile_enable(int on) {
+  clear_error();
+  std::lock_guard<std::mutex> lock(g_prof_mutex);
+  g_prof_on = on != 0;
+  return FC_OK;
+}
+
+int fc_profile_reset(void) {
+  clear_error();
+  std::lock_guard<std::mutex> lock(g_prof_mutex);
+  int rc = prof_drain();
+  g_prof_totals.clear();
+  return rc;
+}
+
+int fc_profile_read(const char* kernel, long long* count, double* total_ms) {
+  clear_error();
+  FC_CHECK_ARG(kernel && count && total_ms, "null argument");
+  std::lock_guard<std::mutex> lock(g_prof_mutex);
+  int rc = prof_drain();
+  if (rc) return rc;
+  *count = 0;
+  *total_ms = 0.0;
+  for (auto& kv : g_prof_totals) {
+    if (kv.first == kernel) {
+      *count = kv.second.first;
+      *total_ms = kv.second.second;
+    }
+  }
+  return FC_OK;
 }
 
 int fc_set_tile_limits(long long max_voxels, int max_views) {

@@ -165,6 +165,21 @@ int fc_release_workspace(int device);
  * 0 = no limit (default: on a 288 GB device the configurations fit whole). Process-wide. */
 int fc_set_tile_limits(long long max_voxels, int max_views);
 
+/* Gather-roofline microbenchmark: the sustained rate (samples/s) of the kernels' own fetch +
+ * interpolation code at pseudo-random positions inside small cache-resident bricks.
+ * which = 0: projector trilinear sample, octet layout of `dtype` (8 taps per sample);
+ *         1: projector trilinear sample, plain FP32 layout; 2: backprojector bilinear quad gather
+ *         of `dtype` (4 taps per sample). Best over several brick sizes. Synchronous. */
+int fc_gather_bench(int device, int which, int dtype, double* samples_per_s);
+
+/* Kernel timing. When enabled, the main kernel of every fc_* / GPU_CB_* call ("project",
+ * "adjoint", "backproject", ...) is bracketed by HIP events recorded on the stream it is launched
+ * on. fc_profile_read waits for those events and returns the launch count and the summed kernel
+ * time (ms) of one kernel name since the last fc_profile_reset. */
+int fc_profile_enable(int on);
+int fc_profile_reset(void);
+int fc_profile_read(const char* kernel, long long* count, double* total_ms);
+
 #ifdef __cplusplus
 }
 #endif
