@@ -87,6 +87,14 @@ __device__ __forceinline__ uint32_t pack2(float a, float b) {
 
 __device__ __forceinline__ float lerp(float a, float b, float t) { return fmaf(t, b - a, a); }
 
+// a[23:0] * b[23:0] + c (low 32 bits), one full-rate VALU instruction. The compiler lowers the
+// equivalent C expression with a variable multiplier to the multi-pass v_mad_u64_u32.
+__device__ __forceinline__ uint32_t mad_u24(uint32_t a, uint32_t b, uint32_t c) {
+  uint32_t r;
+  asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
 // Logical block index for a 1-D grid, remapped so each of the 8 XCDs (which receive blocks
 // round-robin) works on one contiguous range of logical blocks.
 __device__ __forceinline__ int xcd_remap(int b, int nb) {
@@ -106,13 +114,13 @@ struct RaySeg {
   int nk;
 };
 
-__device__ __forceinline__ void ray_dir(const double* V, int j, int i, double* dx, double* dy,
-                                        double* dz) {
+__device__ __forceinline__ void ray_dir(const double* V, double j, double i, double* dx,
+                                        double* dy, double* dz) {
   if (V[24] == (double)FC_DET_CYLINDER) {
-    double g = ((double)j - V[25]) * V[26];
+    double g = (j - V[25]) * V[26];
     double sg, cg;
     sincos(g, &sg, &cg);
-    double r = (double)i - V[27];
+    double r = i - V[27];
     *dx = cg * V[3] + sg * V[6] + r * V[9];
     *dy = cg * V[4] + sg * V[7] + r * V[10];
     *dz = cg * V[5] + sg * V[8] + r * V[11];
@@ -131,7 +139,7 @@ __device__ __forceinline__ RaySeg ray_segment(const double* V, int j, int i, dou
   r.px = r.py = r.pz = 0.f;
   r.sx = r.sy = r.sz = 0.f;
   double dx, dy, dz;
-  ray_dir(V, j, i, &dx, &dy, &dz);
+  ray_dir(V, (double)j, (double)i, &dx, &dy, &dz);
   double len = sqrt(dx * dx + dy * dy + dz * dz);
   if (!(len > 0.0)) return r;
   double inv = 1.0 / len;

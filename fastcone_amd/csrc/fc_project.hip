@@ -27,16 +27,20 @@
 namespace fc {
 namespace {
 
-constexpr int kTile = 16;  // 16x16 pixel tile per 256-thread block, 8x8 per wave
+constexpr int kTile = 16;     // 16x16 pixel tile per 256-thread block, 8x8 per wave
+constexpr int kLdsCap = 8192;  // projector LDS brick budget, floats per block (32 KiB)
 
 typedef float f32x2_a4 __attribute__((ext_vector_type(2), aligned(4)));
 
-enum FpLayout { FP_OCTET = 0, FP_PLAIN = 1 };
+enum FpLayout { FP_OCTET = 0, FP_PLAIN = 1, FP_LDS = 2 };
 
+// FASTCONE_FP_LAYOUT selects the projector variant: "lds" (default: LDS-staged slab bricks over
+// the plain volume), "octet" (global gathers of 8-corner records) or "plain" (global gathers).
 int fp_layout_choice() {
   const char* e = std::getenv("FASTCONE_FP_LAYOUT");
   if (e && std::strcmp(e, "plain") == 0) return FP_PLAIN;
-  return FP_OCTET;
+  if (e && std::strcmp(e, "octet") == 0) return FP_OCTET;
+  return FP_LDS;
 }
 
 // ---- staging ---------------------------------------------------------------------------------
@@ -83,6 +87,20 @@ __global__ __launch_bounds__(256) void stage_plain(const void* __restrict__ vol,
       in ? load_elem<DT_IN>(vol, ((size_t)z * ny + y) * nx + x) : 0.0f;
 }
 
+// Plain 16-bit copy with a zero halo of 2 (raw bits of the input storage type).
+template <int DT_IN>
+__global__ __launch_bounds__(256) void stage_plain16(const void* __restrict__ vol, int nx, int ny,
+                                                     int nz, uint16_t* __restrict__ out, int PX,
+                                                     int PY) {
+  int px = blockIdx.x * 256 + threadIdx.x;
+  int py = blockIdx.y, pz = blockIdx.z;
+  if (px >= PX) return;
+  int x = px - 2, y = py - 2, z = pz - 2;
+  bool in = x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz;
+  out[((size_t)pz * PY + py) * PX + px] =
+      in ? static_cast<const uint16_t*>(vol)[((size_t)z * ny + y) * nx + x] : (uint16_t)0;
+}
+
 // ---- projector -------------------------------------------------------------------------------
 
 struct FpArgs {
@@ -97,23 +115,34 @@ struct FpArgs {
   int nw, nh, nviews;
   int tiles_w, tiles_per_view, group;
   int max_steps;
+  int lds_cap;  // LDS brick budget in floats (<= kLdsCap)
 };
 
 // Tile order: groups of `group` consecutive views; inside a group the same pixel tile of all
 // its views is adjacent, so concurrently running blocks of one XCD gather the same voxels.
-__device__ __forceinline__ void fp_tile(const FpArgs& a, int* t, int* j, int* i) {
+// Returns the view t and the tile's first column / row (j0, i0); the thread's pixel is
+// (j0 + (wave & 1) * 8 + (lane & 7), i0 + (wave >> 1) * 8 + (lane >> 3)).
+__device__ __forceinline__ void fp_tile_origin(int tiles_w, int tiles_per_view, int group,
+                                               int nviews, int* t, int* j0, int* i0) {
   int lb = xcd_remap(blockIdx.x, gridDim.x);
-  int per_group = a.tiles_per_view * a.group;
+  int per_group = tiles_per_view * group;
   int g = lb / per_group;
   int r = lb - g * per_group;
-  int gsize = min(a.group, a.nviews - g * a.group);
+  int gsize = min(group, nviews - g * group);
   int tile = r / gsize;
   int v = r - tile * gsize;
-  *t = g * a.group + v;
-  int ty = tile / a.tiles_w, tx = tile - ty * a.tiles_w;
+  *t = g * group + v;
+  int ty = tile / tiles_w, tx = tile - ty * tiles_w;
+  *j0 = tx * kTile;
+  *i0 = ty * kTile;
+}
+
+__device__ __forceinline__ void fp_tile(const FpArgs& a, int* t, int* j, int* i) {
+  int j0, i0;
+  fp_tile_origin(a.tiles_w, a.tiles_per_view, a.group, a.nviews, t, &j0, &i0);
   int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  *j = tx * kTile + (wave & 1) * 8 + (lane & 7);
-  *i = ty * kTile + (wave >> 1) * 8 + (lane >> 3);
+  *j = j0 + (wave & 1) * 8 + (lane & 7);
+  *i = i0 + (wave >> 1) * 8 + (lane >> 3);
 }
 
 template <int DT, int LAYOUT>
@@ -134,6 +163,12 @@ __device__ __forceinline__ float trilinear_sample(const FpArgs& a, float qx, flo
       v0 = lo16<DT>(w.x); v1 = hi16<DT>(w.x); v2 = lo16<DT>(w.y); v3 = hi16<DT>(w.y);
       v4 = lo16<DT>(w.z); v5 = hi16<DT>(w.z); v6 = lo16<DT>(w.w); v7 = hi16<DT>(w.w);
     }
+  } else if constexpr (DT != FC_F32) {
+    size_t sy = (size_t)a.PX, sz = (size_t)a.PX * a.PY;
+    v0 = load_elem<DT>(a.data, o); v1 = load_elem<DT>(a.data, o + 1);
+    v2 = load_elem<DT>(a.data, o + sy); v3 = load_elem<DT>(a.data, o + sy + 1);
+    v4 = load_elem<DT>(a.data, o + sz); v5 = load_elem<DT>(a.data, o + sz + 1);
+    v6 = load_elem<DT>(a.data, o + sz + sy); v7 = load_elem<DT>(a.data, o + sz + sy + 1);
   } else {
     const float* p = static_cast<const float*>(a.data) + o;
     size_t sy = (size_t)a.PX, sz = (size_t)a.PX * a.PY;
@@ -166,6 +201,178 @@ __global__ __launch_bounds__(256) void fp_kernel(FpArgs a) {
   float scale = (a.norm ? a.norm[t] : 1.0f) * (float)a.h;
   size_t idx = ((size_t)t * a.nh + i) * a.nw + j;
   a.proj[idx] += scale * acc;
+}
+
+// ---- LDS-staged projector ------------------------------------------------------------------------
+//
+// A block owns a 16x16 tile of rays of one view. The tile's rays are marched slab by slab along
+// the volume axis most aligned with the tile's central ray (the "dominant" axis). For each slab of
+// S voxel planes the block stages, in LDS, the brick that contains every trilinear corner the
+// tile's samples in that slab can touch, then each thread evaluates its ray's samples from LDS:
+//   * the brick's extent in the two minor axes is the bounding box of the tile's four corner rays
+//     between the slab planes (the tile's frustum slice), widened by one voxel each side;
+//   * along the dominant axis it spans S + 3 planes, [lo - 1, lo + S + 1];
+//   * LDS layout: dominant axis fastest with compile-time pitch S + 3, so the 8 corners of a
+//     sample are two base addresses with immediate offsets {0, 1, S+3, S+4};
+//   * a ray's samples are split between slabs by integer k ranges computed from the shared slab
+//     planes, so every sample is evaluated exactly once.
+// Bricks larger than the LDS budget (or tiles whose rays do not share a dominant direction) fall
+// back to gathering from the staged volume in global memory, with the same arithmetic.
+
+template <int DT, int S>
+__global__ __launch_bounds__(256) void fp_lds_kernel(FpArgs a) {
+  constexpr int DP = S + 3;  // dominant-axis pitch of the LDS brick
+  __shared__ float brick[kLdsCap];
+  int t, j0, i0;
+  fp_tile_origin(a.tiles_w, a.tiles_per_view, a.group, a.nviews, &t, &j0, &i0);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j = j0 + (wave & 1) * 8 + (lane & 7);
+  const int i = i0 + (wave >> 1) * 8 + (lane >> 3);
+  const bool pix = (t < a.nviews) && (j < a.nw) && (i < a.nh);
+  const double* V = a.views + (size_t)min(t, a.nviews - 1) * FC_VIEW_DOUBLES;
+  RaySeg r;
+  r.nk = 0;
+  r.px = r.py = r.pz = r.sx = r.sy = r.sz = 0.f;
+  if (pix) r = ray_segment(V, j, i, a.ox, a.oy, a.oz, a.nx, a.ny, a.nz, a.h, a.max_steps, 2.0f);
+
+  // dominant axis of the tile's central ray (identical on every thread)
+  const double jc = fmin((double)j0 + 7.5, (double)(a.nw - 1));
+  const double ic = fmin((double)i0 + 7.5, (double)(a.nh - 1));
+  double cdx, cdy, cdz;
+  ray_dir(V, jc, ic, &cdx, &cdy, &cdz);
+  int ax = 0;
+  double best = fabs(cdx);
+  if (fabs(cdy) > best) { ax = 1; best = fabs(cdy); }
+  if (fabs(cdz) > best) { ax = 2; }
+  const double cdom = ax == 0 ? cdx : (ax == 1 ? cdy : cdz);
+  const float dir = cdom > 0.0 ? 1.0f : -1.0f;
+  const int a1 = ax == 2 ? 0 : ax + 1, a2 = ax == 0 ? 2 : (ax == 1 ? 0 : 1);
+
+  auto pick = [](int k, float x, float y, float z) { return k == 0 ? x : (k == 1 ? y : z); };
+  const float pd0 = pick(ax, r.px, r.py, r.pz), p10 = pick(a1, r.px, r.py, r.pz),
+              p20 = pick(a2, r.px, r.py, r.pz);
+  const float sd = pick(ax, r.sx, r.sy, r.sz), s1 = pick(a1, r.sx, r.sy, r.sz),
+              s2 = pick(a2, r.sx, r.sy, r.sz);
+  const int nk = r.nk;
+  // every active ray must advance along the dominant axis in the same direction
+  const bool bad = nk > 0 && !(sd * dir > 0.2f * (float)a.h);
+  const bool fallback = __syncthreads_or(bad);
+
+  float acc = 0.0f;
+  if (fallback) {
+    for (int k = 0; k < nk; ++k) {
+      float kf = (float)k;
+      acc += trilinear_sample<DT, FP_PLAIN>(a, fmaf(kf, r.sx, r.px), fmaf(kf, r.sy, r.py),
+                                            fmaf(kf, r.sz, r.pz));
+    }
+  } else {
+    // padded-frame dims and strides, permuted to (dominant, minor1, minor2)
+    const int PZ = a.nz + 4;
+    const int dimv[3] = {a.PX, a.PY, PZ};
+    const size_t strv[3] = {1, (size_t)a.PX, (size_t)a.PX * a.PY};
+    const int PD = dimv[ax], P1 = dimv[a1], P2 = dimv[a2];
+    const size_t gsd = strv[ax], gs1 = strv[a1], gs2 = strv[a2];
+    const float inv_sd = 1.0f / sd;
+
+    // corner rays of the tile (pixels clamped to the detector): q(P) = src + (P - src_d) * rc
+    const double sx = V[0] - a.ox + 2.0, sy = V[1] - a.oy + 2.0, sz = V[2] - a.oz + 2.0;
+    const double srcv[3] = {sx, sy, sz};
+    const float src_d = (float)srcv[ax], src_1 = (float)srcv[a1], src_2 = (float)srcv[a2];
+    float rc1[4], rc2[4];
+    const double jA = (double)j0, jB = (double)min(j0 + kTile - 1, a.nw - 1);
+    const double iA = (double)i0, iB = (double)min(i0 + kTile - 1, a.nh - 1);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      double dx, dy, dz;
+      ray_dir(V, (c & 1) ? jB : jA, (c & 2) ? iB : iA, &dx, &dy, &dz);
+      const double dv[3] = {dx, dy, dz};
+      double rd = 1.0 / dv[ax];
+      rc1[c] = (float)(dv[a1] * rd);
+      rc2[c] = (float)(dv[a2] * rd);
+    }
+    const float margin = V[24] == (double)FC_DET_CYLINDER ? 2.0f : 1.0f;
+
+    const int nd = (dimv[ax] - 4) + 1;       // support length along the dominant axis
+    const int nslabs = nd / S + 1;           // slabs [1 + b S, 1 + (b+1) S)
+    for (int m = 0; m < nslabs; ++m) {
+      const int b = dir > 0.0f ? m : nslabs - 1 - m;
+      const float lo = 1.0f + (float)(b * S), hi = lo + (float)S;
+      const float Pin = dir > 0.0f ? lo : hi, Pout = dir > 0.0f ? hi : lo;
+      // this ray's samples in the slab
+      int kin = (int)ceilf((Pin - pd0) * inv_sd);
+      int kout = (int)ceilf((Pout - pd0) * inv_sd);
+      kin = min(max(kin, 0), nk);
+      kout = min(max(kout, 0), nk);
+      // frustum slice bounding box in the minor axes (identical on every thread)
+      float mn1 = 3e38f, mx1 = -3e38f, mn2 = 3e38f, mx2 = -3e38f;
+      const float tlo = lo - src_d, thi = hi - src_d;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        float u1 = fmaf(tlo, rc1[c], src_1), v1 = fmaf(thi, rc1[c], src_1);
+        float u2 = fmaf(tlo, rc2[c], src_2), v2 = fmaf(thi, rc2[c], src_2);
+        mn1 = fminf(mn1, fminf(u1, v1)); mx1 = fmaxf(mx1, fmaxf(u1, v1));
+        mn2 = fminf(mn2, fminf(u2, v2)); mx2 = fmaxf(mx2, fmaxf(u2, v2));
+      }
+      // clamp in float first (rays nearly parallel to a slab give huge values)
+      mn1 = fminf(fmaxf(mn1, -4.0f), (float)P1 + 4.0f); mx1 = fminf(fmaxf(mx1, -4.0f), (float)P1 + 4.0f);
+      mn2 = fminf(fmaxf(mn2, -4.0f), (float)P2 + 4.0f); mx2 = fminf(fmaxf(mx2, -4.0f), (float)P2 + 4.0f);
+      const int e1lo = max((int)floorf(mn1 - margin), 0);
+      const int e1hi = min((int)floorf(mx1 + margin) + 1, P1 - 1);
+      const int e2lo = max((int)floorf(mn2 - margin), 0);
+      const int e2hi = min((int)floorf(mx2 + margin) + 1, P2 - 1);
+      const int E1 = e1hi - e1lo + 1, E2 = e2hi - e2lo + 1;
+      if (E1 <= 0 || E2 <= 0) continue;  // the tile's frustum misses the block in this slab
+      const int dlo = (int)lo - 1;
+      if (DP * E1 * E2 > a.lds_cap) {
+        for (int k = kin; k < kout; ++k) {
+          float kf = (float)k;
+          acc += trilinear_sample<DT, FP_PLAIN>(a, fmaf(kf, r.sx, r.px), fmaf(kf, r.sy, r.py),
+                                                fmaf(kf, r.sz, r.pz));
+        }
+        continue;
+      }
+      __syncthreads();  // the previous brick is no longer read
+      const int PS = DP * E1;
+      const size_t g00 = (size_t)e1lo * gs1 + (size_t)e2lo * gs2;
+      for (int p = threadIdx.x; p < PS; p += 256) {
+        const int d = p % DP, m1 = p / DP;
+        const int gd = dlo + d;
+        if (gd >= PD) continue;  // beyond the padded block: never sampled
+        size_t g = g00 + (size_t)gd * gsd + (size_t)m1 * gs1;
+        for (int m2 = 0; m2 < E2; ++m2) {
+          brick[p + PS * m2] = load_elem<DT>(a.data, g);
+          g += gs2;
+        }
+      }
+      __syncthreads();
+      // Brick-relative LDS byte offset of a sample's corner (0,0,0). Sample coordinates are >= 0
+      // in the padded frame (ray_segment guarantees it), so truncation is floor; the 24-bit
+      // multiply-adds stay full rate.
+      const uint32_t E1u = (uint32_t)E1 & 0xffffffu;
+      const uint32_t boff = (uint32_t)(dlo + DP * (e1lo + E1 * e2lo));
+      const uint32_t zbytes = 4u * (uint32_t)PS;
+      const char* base = reinterpret_cast<const char*>(brick) - 4u * boff;
+      float kf = (float)kin;
+      for (int k = kin; k < kout; ++k, kf += 1.0f) {
+        const float qd = fmaf(kf, sd, pd0), q1 = fmaf(kf, s1, p10), q2 = fmaf(kf, s2, p20);
+        const uint32_t id = (uint32_t)(int)qd, i1 = (uint32_t)(int)q1, i2 = (uint32_t)(int)q2;
+        const float ad = __builtin_amdgcn_fractf(qd), a1f = __builtin_amdgcn_fractf(q1),
+                    a2f = __builtin_amdgcn_fractf(q2);
+        const uint32_t row = mad_u24(i2, E1u, i1);
+        const float* L0 = reinterpret_cast<const float*>(base + 4u * mad_u24(row, (uint32_t)DP, id));
+        const float* L1 = reinterpret_cast<const float*>(reinterpret_cast<const char*>(L0) + zbytes);
+        float c00 = lerp(L0[0], L0[1], ad), c10 = lerp(L0[DP], L0[DP + 1], ad);
+        float c01 = lerp(L1[0], L1[1], ad), c11 = lerp(L1[DP], L1[DP + 1], ad);
+        float c0 = lerp(c00, c10, a1f), c1 = lerp(c01, c11, a1f);
+        acc += lerp(c0, c1, a2f);
+      }
+    }
+  }
+  if (pix) {
+    float scale = (a.norm ? a.norm[t] : 1.0f) * (float)a.h;
+    size_t idx = ((size_t)t * a.nh + i) * a.nw + j;
+    a.proj[idx] += scale * acc;
+  }
 }
 
 // ---- exact transpose (scatter) ---------------------------------------------------------------
@@ -311,7 +518,8 @@ int launch_project(int device, const void* vol, int dt, const int vs[3], const d
                    float* proj, const int ps[3], const double* views, double h,
                    const float* norm, hipStream_t stream) {
   const int nx = vs[0], ny = vs[1], nz = vs[2];
-  const int layout = (dt == FC_F32) ? fp_layout_choice() : FP_OCTET;
+  int layout = fp_layout_choice();
+  if (layout == FP_PLAIN && dt != FC_F32) layout = FP_LDS;
   int PX, PY, PZ;
   size_t bytes;
   if (layout == FP_OCTET) {
@@ -319,7 +527,7 @@ int launch_project(int device, const void* vol, int dt, const int vs[3], const d
     bytes = (size_t)PX * PY * PZ * 8 * dtype_size(dt);
   } else {
     PX = nx + 4; PY = ny + 4; PZ = nz + 4;
-    bytes = (size_t)PX * PY * PZ * 4;
+    bytes = (size_t)PX * PY * PZ * (layout == FP_LDS ? dtype_size(dt) : 4);
   }
   FC_CHECK_ARG(PY <= 65535 && PZ <= 65535, "volume dimension too large for staging grid");
   FC_CHECK_ARG((size_t)PX * PY * PZ < (1ull << 32), "volume too large for 32-bit voxel offsets");
@@ -331,8 +539,12 @@ int launch_project(int device, const void* vol, int dt, const int vs[3], const d
     if (dt == FC_F32) stage_octets<FC_F32><<<sg, 256, 0, stream>>>(vol, nx, ny, nz, ws, PX, PY);
     else if (dt == FC_BF16) stage_octets<FC_BF16><<<sg, 256, 0, stream>>>(vol, nx, ny, nz, ws, PX, PY);
     else stage_octets<FC_F16><<<sg, 256, 0, stream>>>(vol, nx, ny, nz, ws, PX, PY);
-  } else {
+  } else if (layout == FP_PLAIN || dt == FC_F32) {
     stage_plain<FC_F32><<<sg, 256, 0, stream>>>(vol, nx, ny, nz, (float*)ws, PX, PY);
+  } else if (dt == FC_BF16) {
+    stage_plain16<FC_BF16><<<sg, 256, 0, stream>>>(vol, nx, ny, nz, (uint16_t*)ws, PX, PY);
+  } else {
+    stage_plain16<FC_F16><<<sg, 256, 0, stream>>>(vol, nx, ny, nz, (uint16_t*)ws, PX, PY);
   }
   FC_HIP(hipGetLastError());
   FpArgs a;
@@ -351,6 +563,9 @@ int launch_project(int device, const void* vol, int dt, const int vs[3], const d
   a.group = g ? std::max(1, std::atoi(g)) : 8;
   a.group = std::min(a.group, std::max(1, a.nviews));
   a.max_steps = max_steps_for(vs, h);
+  // FASTCONE_FP_LDS_CAP lowers the LDS brick budget (testing the global-gather fallback)
+  const char* cap = std::getenv("FASTCONE_FP_LDS_CAP");
+  a.lds_cap = cap ? std::max(0, std::min(kLdsCap, std::atoi(cap))) : kLdsCap;
   long long nblocks = (long long)a.tiles_per_view * a.nviews;
   FC_CHECK_ARG(nblocks < (1ll << 31), "too many projection tiles");
   if (nblocks > 0) {
@@ -359,8 +574,12 @@ int launch_project(int device, const void* vol, int dt, const int vs[3], const d
       if (dt == FC_F32) fp_kernel<FC_F32, FP_OCTET><<<(int)nblocks, 256, 0, stream>>>(a);
       else if (dt == FC_BF16) fp_kernel<FC_BF16, FP_OCTET><<<(int)nblocks, 256, 0, stream>>>(a);
       else fp_kernel<FC_F16, FP_OCTET><<<(int)nblocks, 256, 0, stream>>>(a);
-    } else {
+    } else if (layout == FP_PLAIN) {
       fp_kernel<FC_F32, FP_PLAIN><<<(int)nblocks, 256, 0, stream>>>(a);
+    } else {
+      if (dt == FC_F32) fp_lds_kernel<FC_F32, 8><<<(int)nblocks, 256, 0, stream>>>(a);
+      else if (dt == FC_BF16) fp_lds_kernel<FC_BF16, 8><<<(int)nblocks, 256, 0, stream>>>(a);
+      else fp_lds_kernel<FC_F16, 8><<<(int)nblocks, 256, 0, stream>>>(a);
     }
     prof_end(tok, stream);
     FC_HIP(hipGetLastError());

@@ -58,12 +58,28 @@ def test_project_f32_parity(gpu, h, mis):
     assert rel(p, ref) < 5e-5
 
 
-@pytest.mark.parametrize("layout", ["octet", "plain"])
-def test_project_layouts_agree(gpu, layout, monkeypatch):
+@pytest.mark.parametrize("layout,cap", [("octet", None), ("plain", None), ("lds", None),
+                                        ("lds", "600"), ("lds", "0")])
+def test_project_layouts_agree(gpu, layout, cap, monkeypatch):
+    # cap lowers the LDS brick budget so some (600) or all (0) slabs take the global fallback
     from fastcone_amd import ops
     monkeypatch.setenv("FASTCONE_FP_LAYOUT", layout)
+    if cap is not None:
+        monkeypatch.setenv("FASTCONE_FP_LDS_CAP", cap)
     rng = np.random.default_rng(12)
     g, prob = _setup(rng)
+    x = rng.standard_normal(g.vol_shape).astype(np.float32)
+    p = ops.project(torch.from_numpy(x).to(gpu), g, step=0.5).cpu()
+    ref = oracle.project(prob, x, window=g.det_shape[::-1], h=0.5)
+    assert rel(p, ref) < 5e-5
+
+
+def test_project_lds_steep_and_wide_cone(gpu):
+    # short source distance and a tilted detector: large cone angles, views whose dominant axis
+    # is z for some tiles, and bricks that exceed the LDS budget
+    from fastcone_amd import ops
+    rng = np.random.default_rng(14)
+    g, prob = _setup(rng, D=28.0, pw=1.6, mis=[0.0, 0.0, 0.0, 0.9, 0.3, 0.2])
     x = rng.standard_normal(g.vol_shape).astype(np.float32)
     p = ops.project(torch.from_numpy(x).to(gpu), g, step=0.5).cpu()
     ref = oracle.project(prob, x, window=g.det_shape[::-1], h=0.5)
