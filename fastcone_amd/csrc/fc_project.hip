@@ -219,6 +219,127 @@ __global__ __launch_bounds__(256) void fp_kernel(FpArgs a) {
 // Bricks larger than the LDS budget (or tiles whose rays do not share a dominant direction) fall
 // back to gathering from the staged volume in global memory, with the same arithmetic.
 
+// Per-block slab plan shared by the LDS projector and the LDS adjoint: the dominant axis of the
+// tile's central ray, the permuted first sample / step of this thread's ray, and the tile's four
+// corner rays (for the frustum-slice boxes). Every field except the ray's is identical on all
+// threads of the block (same inputs, same instructions).
+struct SlabPlan {
+  int ax, a1, a2;
+  float dir;
+  float pd0, p10, p20, sd, s1, s2, inv_sd;
+  int nk;
+  int PD, P1, P2;
+  uint32_t gsd, gs1, gs2;
+  float src_d, src_1, src_2;
+  float rc1[4], rc2[4];
+  float margin;
+  int nslabs;
+};
+
+template <int S>
+__device__ __forceinline__ void make_plan(SlabPlan& P, const double* V, const RaySeg& r, int j0,
+                                          int i0, int nw, int nh, double ox, double oy, double oz,
+                                          int PX, int PY, int nz) {
+  const double jc = fmin((double)j0 + 7.5, (double)(nw - 1));
+  const double ic = fmin((double)i0 + 7.5, (double)(nh - 1));
+  double cdx, cdy, cdz;
+  ray_dir(V, jc, ic, &cdx, &cdy, &cdz);
+  int ax = 0;
+  double best = fabs(cdx);
+  if (fabs(cdy) > best) { ax = 1; best = fabs(cdy); }
+  if (fabs(cdz) > best) { ax = 2; }
+  const double cdom = ax == 0 ? cdx : (ax == 1 ? cdy : cdz);
+  P.ax = ax;
+  P.dir = cdom > 0.0 ? 1.0f : -1.0f;
+  P.a1 = ax == 2 ? 0 : ax + 1;
+  P.a2 = ax == 0 ? 2 : (ax == 1 ? 0 : 1);
+  auto pick = [](int k, float x, float y, float z) { return k == 0 ? x : (k == 1 ? y : z); };
+  P.pd0 = pick(ax, r.px, r.py, r.pz);
+  P.p10 = pick(P.a1, r.px, r.py, r.pz);
+  P.p20 = pick(P.a2, r.px, r.py, r.pz);
+  P.sd = pick(ax, r.sx, r.sy, r.sz);
+  P.s1 = pick(P.a1, r.sx, r.sy, r.sz);
+  P.s2 = pick(P.a2, r.sx, r.sy, r.sz);
+  P.inv_sd = 1.0f / P.sd;
+  P.nk = r.nk;
+  const int PZ = nz + 4;
+  const int dimv[3] = {PX, PY, PZ};
+  const uint32_t strv[3] = {1u, (uint32_t)PX, (uint32_t)PX * (uint32_t)PY};
+  P.PD = dimv[ax]; P.P1 = dimv[P.a1]; P.P2 = dimv[P.a2];
+  P.gsd = strv[ax]; P.gs1 = strv[P.a1]; P.gs2 = strv[P.a2];
+  const double srcv[3] = {V[0] - ox + 2.0, V[1] - oy + 2.0, V[2] - oz + 2.0};
+  P.src_d = (float)srcv[ax];
+  P.src_1 = (float)srcv[P.a1];
+  P.src_2 = (float)srcv[P.a2];
+  const double jA = (double)j0, jB = (double)min(j0 + kTile - 1, nw - 1);
+  const double iA = (double)i0, iB = (double)min(i0 + kTile - 1, nh - 1);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    double dx, dy, dz;
+    ray_dir(V, (c & 1) ? jB : jA, (c & 2) ? iB : iA, &dx, &dy, &dz);
+    const double dv[3] = {dx, dy, dz};
+    const double rd = 1.0 / dv[ax];
+    P.rc1[c] = (float)(dv[P.a1] * rd);
+    P.rc2[c] = (float)(dv[P.a2] * rd);
+  }
+  P.margin = V[24] == (double)FC_DET_CYLINDER ? 2.0f : 1.0f;
+  P.nslabs = (P.PD - 4 + 1) / S + 1;  // slabs [1 + b S, 1 + (b+1) S) cover [1, n + 2]
+}
+
+// Block-uniform box of slab m (traversal order) and this ray's k range in it.
+struct SlabBox {
+  int kin, kout;
+  int dlo, e1lo, e2lo, E1, E2;
+};
+
+template <int S>
+__device__ __forceinline__ SlabBox slab_box(const SlabPlan& P, int m) {
+  SlabBox B;
+  const int b = P.dir > 0.0f ? m : P.nslabs - 1 - m;
+  const float lo = 1.0f + (float)(b * S), hi = lo + (float)S;
+  const float Pin = P.dir > 0.0f ? lo : hi, Pout = P.dir > 0.0f ? hi : lo;
+  int kin = (int)ceilf((Pin - P.pd0) * P.inv_sd);
+  int kout = (int)ceilf((Pout - P.pd0) * P.inv_sd);
+  B.kin = min(max(kin, 0), P.nk);
+  B.kout = min(max(kout, 0), P.nk);
+  float mn1 = 3e38f, mx1 = -3e38f, mn2 = 3e38f, mx2 = -3e38f;
+  const float tlo = lo - P.src_d, thi = hi - P.src_d;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    float u1 = fmaf(tlo, P.rc1[c], P.src_1), v1 = fmaf(thi, P.rc1[c], P.src_1);
+    float u2 = fmaf(tlo, P.rc2[c], P.src_2), v2 = fmaf(thi, P.rc2[c], P.src_2);
+    mn1 = fminf(mn1, fminf(u1, v1)); mx1 = fmaxf(mx1, fmaxf(u1, v1));
+    mn2 = fminf(mn2, fminf(u2, v2)); mx2 = fmaxf(mx2, fmaxf(u2, v2));
+  }
+  // clamp in float first (rays nearly parallel to a slab give huge values)
+  const float P1f = (float)P.P1, P2f = (float)P.P2;
+  mn1 = fminf(fmaxf(mn1, -4.0f), P1f + 4.0f); mx1 = fminf(fmaxf(mx1, -4.0f), P1f + 4.0f);
+  mn2 = fminf(fmaxf(mn2, -4.0f), P2f + 4.0f); mx2 = fminf(fmaxf(mx2, -4.0f), P2f + 4.0f);
+  B.e1lo = max((int)floorf(mn1 - P.margin), 0);
+  const int e1hi = min((int)floorf(mx1 + P.margin) + 1, P.P1 - 1);
+  B.e2lo = max((int)floorf(mn2 - P.margin), 0);
+  const int e2hi = min((int)floorf(mx2 + P.margin) + 1, P.P2 - 1);
+  B.E1 = e1hi - B.e1lo + 1;
+  B.E2 = e2hi - B.e2lo + 1;
+  B.dlo = (int)lo - 1;
+  return B;
+}
+
+// Global element offset of brick element e = d + DP (m1 + E1 m2) (d fastest). Planes past the
+// padded block (last slab) are clamped to its last plane; they are never sampled.
+template <int DP>
+__device__ __forceinline__ uint32_t brick_elem_offset(const SlabPlan& P, const SlabBox& B,
+                                                      uint32_t e, float invE1) {
+  const uint32_t d = e % DP, rest = e / DP;
+  // exact floor(rest / E1) for rest < 2^16: rest + 1/2 keeps the quotient 1/(2 E1) away from
+  // an integer, far above the FP32 rounding of the product
+  const uint32_t m2 = (uint32_t)(((float)rest + 0.5f) * invE1);
+  const uint32_t m1 = rest - m2 * (uint32_t)B.E1;
+  const uint32_t dd = min(d, (uint32_t)(P.PD - 1 - B.dlo));
+  return (uint32_t)B.dlo * P.gsd + (uint32_t)B.e1lo * P.gs1 + (uint32_t)B.e2lo * P.gs2 +
+         dd * P.gsd + m1 * P.gs1 + m2 * P.gs2;
+}
+
 template <int DT, int S>
 __global__ __launch_bounds__(256) void fp_lds_kernel(FpArgs a) {
   constexpr int DP = S + 3;  // dominant-axis pitch of the LDS brick
@@ -234,97 +355,27 @@ __global__ __launch_bounds__(256) void fp_lds_kernel(FpArgs a) {
   r.nk = 0;
   r.px = r.py = r.pz = r.sx = r.sy = r.sz = 0.f;
   if (pix) r = ray_segment(V, j, i, a.ox, a.oy, a.oz, a.nx, a.ny, a.nz, a.h, a.max_steps, 2.0f);
-
-  // dominant axis of the tile's central ray (identical on every thread)
-  const double jc = fmin((double)j0 + 7.5, (double)(a.nw - 1));
-  const double ic = fmin((double)i0 + 7.5, (double)(a.nh - 1));
-  double cdx, cdy, cdz;
-  ray_dir(V, jc, ic, &cdx, &cdy, &cdz);
-  int ax = 0;
-  double best = fabs(cdx);
-  if (fabs(cdy) > best) { ax = 1; best = fabs(cdy); }
-  if (fabs(cdz) > best) { ax = 2; }
-  const double cdom = ax == 0 ? cdx : (ax == 1 ? cdy : cdz);
-  const float dir = cdom > 0.0 ? 1.0f : -1.0f;
-  const int a1 = ax == 2 ? 0 : ax + 1, a2 = ax == 0 ? 2 : (ax == 1 ? 0 : 1);
-
-  auto pick = [](int k, float x, float y, float z) { return k == 0 ? x : (k == 1 ? y : z); };
-  const float pd0 = pick(ax, r.px, r.py, r.pz), p10 = pick(a1, r.px, r.py, r.pz),
-              p20 = pick(a2, r.px, r.py, r.pz);
-  const float sd = pick(ax, r.sx, r.sy, r.sz), s1 = pick(a1, r.sx, r.sy, r.sz),
-              s2 = pick(a2, r.sx, r.sy, r.sz);
-  const int nk = r.nk;
+  SlabPlan P;
+  make_plan<S>(P, V, r, j0, i0, a.nw, a.nh, a.ox, a.oy, a.oz, a.PX, a.PY, a.nz);
   // every active ray must advance along the dominant axis in the same direction
-  const bool bad = nk > 0 && !(sd * dir > 0.2f * (float)a.h);
+  const bool bad = r.nk > 0 && !(P.sd * P.dir > 0.2f * (float)a.h);
   const bool fallback = __syncthreads_or(bad);
 
   float acc = 0.0f;
   if (fallback) {
-    for (int k = 0; k < nk; ++k) {
+    for (int k = 0; k < r.nk; ++k) {
       float kf = (float)k;
       acc += trilinear_sample<DT, FP_PLAIN>(a, fmaf(kf, r.sx, r.px), fmaf(kf, r.sy, r.py),
                                             fmaf(kf, r.sz, r.pz));
     }
   } else {
-    // padded-frame dims and strides, permuted to (dominant, minor1, minor2)
-    const int PZ = a.nz + 4;
-    const int dimv[3] = {a.PX, a.PY, PZ};
-    const size_t strv[3] = {1, (size_t)a.PX, (size_t)a.PX * a.PY};
-    const int PD = dimv[ax], P1 = dimv[a1], P2 = dimv[a2];
-    const size_t gsd = strv[ax], gs1 = strv[a1], gs2 = strv[a2];
-    const float inv_sd = 1.0f / sd;
-
-    // corner rays of the tile (pixels clamped to the detector): q(P) = src + (P - src_d) * rc
-    const double sx = V[0] - a.ox + 2.0, sy = V[1] - a.oy + 2.0, sz = V[2] - a.oz + 2.0;
-    const double srcv[3] = {sx, sy, sz};
-    const float src_d = (float)srcv[ax], src_1 = (float)srcv[a1], src_2 = (float)srcv[a2];
-    float rc1[4], rc2[4];
-    const double jA = (double)j0, jB = (double)min(j0 + kTile - 1, a.nw - 1);
-    const double iA = (double)i0, iB = (double)min(i0 + kTile - 1, a.nh - 1);
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      double dx, dy, dz;
-      ray_dir(V, (c & 1) ? jB : jA, (c & 2) ? iB : iA, &dx, &dy, &dz);
-      const double dv[3] = {dx, dy, dz};
-      double rd = 1.0 / dv[ax];
-      rc1[c] = (float)(dv[a1] * rd);
-      rc2[c] = (float)(dv[a2] * rd);
-    }
-    const float margin = V[24] == (double)FC_DET_CYLINDER ? 2.0f : 1.0f;
-
-    const int nd = (dimv[ax] - 4) + 1;       // support length along the dominant axis
-    const int nslabs = nd / S + 1;           // slabs [1 + b S, 1 + (b+1) S)
-    for (int m = 0; m < nslabs; ++m) {
-      const int b = dir > 0.0f ? m : nslabs - 1 - m;
-      const float lo = 1.0f + (float)(b * S), hi = lo + (float)S;
-      const float Pin = dir > 0.0f ? lo : hi, Pout = dir > 0.0f ? hi : lo;
-      // this ray's samples in the slab
-      int kin = (int)ceilf((Pin - pd0) * inv_sd);
-      int kout = (int)ceilf((Pout - pd0) * inv_sd);
-      kin = min(max(kin, 0), nk);
-      kout = min(max(kout, 0), nk);
-      // frustum slice bounding box in the minor axes (identical on every thread)
-      float mn1 = 3e38f, mx1 = -3e38f, mn2 = 3e38f, mx2 = -3e38f;
-      const float tlo = lo - src_d, thi = hi - src_d;
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        float u1 = fmaf(tlo, rc1[c], src_1), v1 = fmaf(thi, rc1[c], src_1);
-        float u2 = fmaf(tlo, rc2[c], src_2), v2 = fmaf(thi, rc2[c], src_2);
-        mn1 = fminf(mn1, fminf(u1, v1)); mx1 = fmaxf(mx1, fmaxf(u1, v1));
-        mn2 = fminf(mn2, fminf(u2, v2)); mx2 = fmaxf(mx2, fmaxf(u2, v2));
-      }
-      // clamp in float first (rays nearly parallel to a slab give huge values)
-      mn1 = fminf(fmaxf(mn1, -4.0f), (float)P1 + 4.0f); mx1 = fminf(fmaxf(mx1, -4.0f), (float)P1 + 4.0f);
-      mn2 = fminf(fmaxf(mn2, -4.0f), (float)P2 + 4.0f); mx2 = fminf(fmaxf(mx2, -4.0f), (float)P2 + 4.0f);
-      const int e1lo = max((int)floorf(mn1 - margin), 0);
-      const int e1hi = min((int)floorf(mx1 + margin) + 1, P1 - 1);
-      const int e2lo = max((int)floorf(mn2 - margin), 0);
-      const int e2hi = min((int)floorf(mx2 + margin) + 1, P2 - 1);
-      const int E1 = e1hi - e1lo + 1, E2 = e2hi - e2lo + 1;
-      if (E1 <= 0 || E2 <= 0) continue;  // the tile's frustum misses the block in this slab
-      const int dlo = (int)lo - 1;
-      if (DP * E1 * E2 > a.lds_cap) {
-        for (int k = kin; k < kout; ++k) {
+    for (int m = 0; m < P.nslabs; ++m) {
+      const SlabBox B = slab_box<S>(P, m);
+      if (B.E1 <= 0 || B.E2 <= 0) continue;  // the tile's frustum misses the block here
+      const int PS = DP * B.E1;
+      const int NE = PS * B.E2;
+      if (NE > a.lds_cap) {
+        for (int k = B.kin; k < B.kout; ++k) {
           float kf = (float)k;
           acc += trilinear_sample<DT, FP_PLAIN>(a, fmaf(kf, r.sx, r.px), fmaf(kf, r.sy, r.py),
                                                 fmaf(kf, r.sz, r.pz));
@@ -332,40 +383,60 @@ __global__ __launch_bounds__(256) void fp_lds_kernel(FpArgs a) {
         continue;
       }
       __syncthreads();  // the previous brick is no longer read
-      const int PS = DP * E1;
-      const size_t g00 = (size_t)e1lo * gs1 + (size_t)e2lo * gs2;
-      for (int p = threadIdx.x; p < PS; p += 256) {
-        const int d = p % DP, m1 = p / DP;
-        const int gd = dlo + d;
-        if (gd >= PD) continue;  // beyond the padded block: never sampled
-        size_t g = g00 + (size_t)gd * gsd + (size_t)m1 * gs1;
-        for (int m2 = 0; m2 < E2; ++m2) {
-          brick[p + PS * m2] = load_elem<DT>(a.data, g);
-          g += gs2;
+      // Fill: every load of a chunk is issued before its LDS writes (one round trip per chunk).
+      {
+        const float invE1 = 1.0f / (float)B.E1;
+        constexpr int kChunk = 8;
+#pragma unroll 1
+        for (int q0 = 0; q0 < kLdsCap / 256; q0 += kChunk) {
+          if ((int)threadIdx.x + 256 * q0 >= NE) break;
+          float v[kChunk];
+#pragma unroll
+          for (int q = 0; q < kChunk; ++q) {
+            const int e = threadIdx.x + 256 * (q0 + q);
+            v[q] = 0.0f;
+            if (e < NE) v[q] = load_elem<DT>(a.data, brick_elem_offset<DP>(P, B, e, invE1));
+          }
+#pragma unroll
+          for (int q = 0; q < kChunk; ++q) {
+            const int e = threadIdx.x + 256 * (q0 + q);
+            if (e < NE) brick[e] = v[q];
+          }
         }
       }
       __syncthreads();
       // Brick-relative LDS byte offset of a sample's corner (0,0,0). Sample coordinates are >= 0
       // in the padded frame (ray_segment guarantees it), so truncation is floor; the 24-bit
       // multiply-adds stay full rate.
-      const uint32_t E1u = (uint32_t)E1 & 0xffffffu;
-      const uint32_t boff = (uint32_t)(dlo + DP * (e1lo + E1 * e2lo));
-      const uint32_t zbytes = 4u * (uint32_t)PS;
+      const uint32_t E1u = (uint32_t)B.E1;
+      const uint32_t boff = (uint32_t)(B.dlo + DP * (B.e1lo + B.E1 * B.e2lo));
+      const uint32_t zbytes = 4u * (uint32_t)PS;  // one m2 plane
       const char* base = reinterpret_cast<const char*>(brick) - 4u * boff;
-      float kf = (float)kin;
-      for (int k = kin; k < kout; ++k, kf += 1.0f) {
-        const float qd = fmaf(kf, sd, pd0), q1 = fmaf(kf, s1, p10), q2 = fmaf(kf, s2, p20);
+      auto sample = [&](float kf) {
+        const float qd = fmaf(kf, P.sd, P.pd0), q1 = fmaf(kf, P.s1, P.p10),
+                    q2 = fmaf(kf, P.s2, P.p20);
         const uint32_t id = (uint32_t)(int)qd, i1 = (uint32_t)(int)q1, i2 = (uint32_t)(int)q2;
         const float ad = __builtin_amdgcn_fractf(qd), a1f = __builtin_amdgcn_fractf(q1),
                     a2f = __builtin_amdgcn_fractf(q2);
         const uint32_t row = mad_u24(i2, E1u, i1);
-        const float* L0 = reinterpret_cast<const float*>(base + 4u * mad_u24(row, (uint32_t)DP, id));
-        const float* L1 = reinterpret_cast<const float*>(reinterpret_cast<const char*>(L0) + zbytes);
+        const float* L0 =
+            reinterpret_cast<const float*>(base + 4u * mad_u24(row, (uint32_t)DP, id));
+        const float* L1 =
+            reinterpret_cast<const float*>(reinterpret_cast<const char*>(L0) + zbytes);
         float c00 = lerp(L0[0], L0[1], ad), c10 = lerp(L0[DP], L0[DP + 1], ad);
         float c01 = lerp(L1[0], L1[1], ad), c11 = lerp(L1[DP], L1[DP + 1], ad);
         float c0 = lerp(c00, c10, a1f), c1 = lerp(c01, c11, a1f);
-        acc += lerp(c0, c1, a2f);
+        return lerp(c0, c1, a2f);
+      };
+      // two samples per iteration: both samples' LDS reads are in flight together
+      float kf = (float)B.kin;
+      int k = B.kin;
+      for (; k + 1 < B.kout; k += 2, kf += 2.0f) {
+        const float r0 = sample(kf), r1 = sample(kf + 1.0f);
+        acc += r0;
+        acc += r1;
       }
+      if (k < B.kout) acc += sample(kf);
     }
   }
   if (pix) {
@@ -389,7 +460,109 @@ struct AdjArgs {
   int nw, nh, nviews;
   int tiles_w, tiles_per_view, group;
   int max_steps;
+  int lds_cap;
 };
+
+// Scatter of one sample's trilinear weights times y into the 8 corners (global atomics).
+__device__ __forceinline__ void scatter_global(float* accp, uint32_t PX, uint32_t PXY, float qx,
+                                               float qy, float qz, float y) {
+  const uint32_t ix = (uint32_t)(int)qx, iy = (uint32_t)(int)qy, iz = (uint32_t)(int)qz;
+  const float ax = __builtin_amdgcn_fractf(qx), ay = __builtin_amdgcn_fractf(qy),
+              az = __builtin_amdgcn_fractf(qz);
+  float* p = accp + ((size_t)iz * PXY + (size_t)iy * PX + ix);
+  const float wz1 = y * az, wz0 = y - wz1;
+  const float w10 = wz0 * ay, w00 = wz0 - w10, w11 = wz1 * ay, w01 = wz1 - w11;
+  float v;
+  v = w00 * ax; unsafeAtomicAdd(p + 1, v); unsafeAtomicAdd(p, w00 - v);
+  v = w10 * ax; unsafeAtomicAdd(p + PX + 1, v); unsafeAtomicAdd(p + PX, w10 - v);
+  v = w01 * ax; unsafeAtomicAdd(p + PXY + 1, v); unsafeAtomicAdd(p + PXY, w01 - v);
+  v = w11 * ax; unsafeAtomicAdd(p + PXY + PX + 1, v); unsafeAtomicAdd(p + PXY + PX, w11 - v);
+}
+
+// LDS-staged exact transpose of fp_lds_kernel: the same tiles, slab plans and per-slab k ranges;
+// each thread scatters its samples' weights times its pixel value into the slab's LDS brick
+// (ds_add_f32), and the brick is then added to the padded global accumulator with one atomic per
+// nonzero element (bricks of different tiles overlap).
+template <int DT, int S>
+__global__ __launch_bounds__(256) void adjoint_lds_kernel(AdjArgs a) {
+  constexpr int DP = S + 3;
+  __shared__ float brick[kLdsCap];
+  int t, j0, i0;
+  fp_tile_origin(a.tiles_w, a.tiles_per_view, a.group, a.nviews, &t, &j0, &i0);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j = j0 + (wave & 1) * 8 + (lane & 7);
+  const int i = i0 + (wave >> 1) * 8 + (lane >> 3);
+  const bool pix = (t < a.nviews) && (j < a.nw) && (i < a.nh);
+  const double* V = a.views + (size_t)min(t, a.nviews - 1) * FC_VIEW_DOUBLES;
+  float y = 0.0f;
+  if (pix) {
+    size_t pidx = ((size_t)t * a.nh + i) * a.nw + j;
+    y = load_elem<DT>(a.proj, pidx) * (a.norm ? a.norm[t] : 1.0f) * (float)a.h;
+  }
+  RaySeg r;
+  r.nk = 0;
+  r.px = r.py = r.pz = r.sx = r.sy = r.sz = 0.f;
+  if (pix) r = ray_segment(V, j, i, a.ox, a.oy, a.oz, a.nx, a.ny, a.nz, a.h, a.max_steps, 2.0f);
+  if (y == 0.0f) r.nk = 0;
+  SlabPlan P;
+  make_plan<S>(P, V, r, j0, i0, a.nw, a.nh, a.ox, a.oy, a.oz, a.PX, a.PY, a.nz);
+  const bool bad = r.nk > 0 && !(P.sd * P.dir > 0.2f * (float)a.h);
+  const bool fallback = __syncthreads_or(bad);
+  const uint32_t PXu = (uint32_t)a.PX, PXY = (uint32_t)a.PX * (uint32_t)a.PY;
+  if (fallback) {
+    for (int k = 0; k < r.nk; ++k) {
+      float kf = (float)k;
+      scatter_global(a.acc, PXu, PXY, fmaf(kf, r.sx, r.px), fmaf(kf, r.sy, r.py),
+                     fmaf(kf, r.sz, r.pz), y);
+    }
+    return;
+  }
+  for (int m = 0; m < P.nslabs; ++m) {
+    const SlabBox B = slab_box<S>(P, m);
+    if (B.E1 <= 0 || B.E2 <= 0) continue;
+    const int PS = DP * B.E1;
+    const int NE = PS * B.E2;
+    if (NE > a.lds_cap) {
+      for (int k = B.kin; k < B.kout; ++k) {
+        float kf = (float)k;
+        scatter_global(a.acc, PXu, PXY, fmaf(kf, r.sx, r.px), fmaf(kf, r.sy, r.py),
+                       fmaf(kf, r.sz, r.pz), y);
+      }
+      continue;
+    }
+    __syncthreads();  // the previous brick has been flushed
+    for (int e = threadIdx.x; e < NE; e += 256) brick[e] = 0.0f;
+    __syncthreads();
+    const uint32_t E1u = (uint32_t)B.E1;
+    const uint32_t boff = (uint32_t)(B.dlo + DP * (B.e1lo + B.E1 * B.e2lo));
+    const uint32_t zfl = (uint32_t)PS;
+    float* base = brick - boff;
+    float kf = (float)B.kin;
+    for (int k = B.kin; k < B.kout; ++k, kf += 1.0f) {
+      const float qd = fmaf(kf, P.sd, P.pd0), q1 = fmaf(kf, P.s1, P.p10),
+                  q2 = fmaf(kf, P.s2, P.p20);
+      const uint32_t id = (uint32_t)(int)qd, i1 = (uint32_t)(int)q1, i2 = (uint32_t)(int)q2;
+      const float ad = __builtin_amdgcn_fractf(qd), a1f = __builtin_amdgcn_fractf(q1),
+                  a2f = __builtin_amdgcn_fractf(q2);
+      float* L0 = base + mad_u24(mad_u24(i2, E1u, i1), (uint32_t)DP, id);
+      float* L1 = L0 + zfl;
+      const float wz1 = y * a2f, wz0 = y - wz1;
+      const float w10 = wz0 * a1f, w00 = wz0 - w10, w11 = wz1 * a1f, w01 = wz1 - w11;
+      float v;
+      v = w00 * ad; atomicAdd(L0 + 1, v); atomicAdd(L0, w00 - v);
+      v = w10 * ad; atomicAdd(L0 + DP + 1, v); atomicAdd(L0 + DP, w10 - v);
+      v = w01 * ad; atomicAdd(L1 + 1, v); atomicAdd(L1, w01 - v);
+      v = w11 * ad; atomicAdd(L1 + DP + 1, v); atomicAdd(L1 + DP, w11 - v);
+    }
+    __syncthreads();
+    const float invE1 = 1.0f / (float)B.E1;
+    for (int e = threadIdx.x; e < NE; e += 256) {
+      const float v = brick[e];
+      if (v != 0.0f && (int)((uint32_t)e % DP) <= P.PD - 1 - B.dlo)
+        unsafeAtomicAdd(a.acc + brick_elem_offset<DP>(P, B, (uint32_t)e, invE1), v);
+    }
+  }
+}
 
 template <int DT>
 __global__ __launch_bounds__(256) void adjoint_kernel(AdjArgs a) {
@@ -615,11 +788,18 @@ int launch_adjoint(int device, float* vol, const int vs[3], const double origin[
   a.tiles_per_view = a.tiles_w * ((a.nh + kTile - 1) / kTile);
   a.group = std::min(8, std::max(1, a.nviews));
   a.max_steps = max_steps_for(vs, h);
+  const char* cap = std::getenv("FASTCONE_FP_LDS_CAP");
+  a.lds_cap = cap ? std::max(0, std::min(kLdsCap, std::atoi(cap))) : kLdsCap;
+  const bool lds = fp_layout_choice() == FP_LDS;
   long long nblocks = (long long)a.tiles_per_view * a.nviews;
   FC_CHECK_ARG(nblocks < (1ll << 31), "too many projection tiles");
   if (nblocks > 0) {
     int tok = prof_begin("adjoint", stream);
-    if (dt == FC_F32) adjoint_kernel<FC_F32><<<(int)nblocks, 256, 0, stream>>>(a);
+    if (lds) {
+      if (dt == FC_F32) adjoint_lds_kernel<FC_F32, 8><<<(int)nblocks, 256, 0, stream>>>(a);
+      else if (dt == FC_BF16) adjoint_lds_kernel<FC_BF16, 8><<<(int)nblocks, 256, 0, stream>>>(a);
+      else adjoint_lds_kernel<FC_F16, 8><<<(int)nblocks, 256, 0, stream>>>(a);
+    } else if (dt == FC_F32) adjoint_kernel<FC_F32><<<(int)nblocks, 256, 0, stream>>>(a);
     else if (dt == FC_BF16) adjoint_kernel<FC_BF16><<<(int)nblocks, 256, 0, stream>>>(a);
     else adjoint_kernel<FC_F16><<<(int)nblocks, 256, 0, stream>>>(a);
     prof_end(tok, stream);

@@ -150,6 +150,31 @@ def test_backproject_accumulates_into_out(gpu):
     assert rel(out.cpu().numpy() - base, ref) < 5e-5
 
 
+@pytest.mark.parametrize("layout,cap", [("lds", None), ("lds", "600"), ("lds", "0"),
+                                        ("plain", None)])
+def test_adjoint_paths_agree(gpu, layout, cap, monkeypatch):
+    from fastcone_amd import ops
+    monkeypatch.setenv("FASTCONE_FP_LAYOUT", layout)
+    if cap is not None:
+        monkeypatch.setenv("FASTCONE_FP_LDS_CAP", cap)
+    rng = np.random.default_rng(32)
+    g, prob = _setup(rng)
+    y = rng.standard_normal(g.proj_shape).astype(np.float32)
+    aty = ops.adjoint(torch.from_numpy(y).to(gpu), g, step=0.5).cpu().numpy()
+    ref = oracle.adjoint(prob, y, g.vol_shape, h=0.5)
+    assert rel(aty, ref) < 5e-5
+
+
+def test_adjoint_steep_and_wide_cone(gpu):
+    from fastcone_amd import ops
+    rng = np.random.default_rng(33)
+    g, prob = _setup(rng, D=28.0, pw=1.6, mis=[0.0, 0.0, 0.0, 0.9, 0.3, 0.2])
+    y = rng.standard_normal(g.proj_shape).astype(np.float32)
+    aty = ops.adjoint(torch.from_numpy(y).to(gpu), g, step=0.5).cpu().numpy()
+    ref = oracle.adjoint(prob, y, g.vol_shape, h=0.5)
+    assert rel(aty, ref) < 5e-5
+
+
 @pytest.mark.parametrize("h", [1.0, 0.5])
 def test_adjoint_parity_and_dot_product(gpu, h):
     from fastcone_amd import ops
