@@ -41,14 +41,16 @@ def _declare(lib):
         "fc_device_count": (i, []),
         "fc_views_flat": (i, [i, _c_dbl_p, _c_dbl_p, d, d, d, _c_dbl_p, i, _c_int_p, _c_int_p,
                               _c_int_p, _c_dbl_p]),
-        "fc_views_cylinder": (i, [i, _c_dbl_p, _c_dbl_p, d, d, d, d, _c_int_p, _c_int_p,
-                                  _c_int_p, _c_dbl_p]),
+        "fc_views_cylinder": (i, [i, _c_dbl_p, _c_dbl_p, d, d, d, d, _c_dbl_p, _c_dbl_p,
+                                  _c_dbl_p]),
         "fc_project_dev": (i, [i, _vp, i, _c_int_p, _c_dbl_p, _vp, _c_int_p, _vp, d, _vp, _vp]),
         "fc_adjoint_dev": (i, [i, _vp, _c_int_p, _c_dbl_p, _vp, i, _c_int_p, _vp, d, _vp, _vp]),
         "fc_backproject_dev": (i, [i, _vp, _c_int_p, _c_dbl_p, _vp, i, _c_int_p, _vp, _vp, i,
                                    _vp]),
         "fc_sample_count_dev": (i, [i, _c_int_p, _c_dbl_p, _c_int_p, _vp, d, _vp, _vp]),
         "fc_convert_dev": (i, [i, _vp, i, _vp, i, ll, _vp]),
+        "fc_katsevich_dev": (i, [i, _vp, i, i, i, i, d, d, d, d, d, d, d, d, d, d, i, _vp,
+                                 _c_int_p, _c_dbl_p, _vp, _vp]),
         "fc_release_workspace": (i, [i]),
         "fc_set_tile_limits": (i, [ll, i]),
         "fc_profile_enable": (i, [i]),

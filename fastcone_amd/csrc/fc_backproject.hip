@@ -249,6 +249,19 @@ int launch_backproject(int device, float* vol, const int vs[3], const double ori
   return release_ws(device, SLOT_QUAD, stream);
 }
 
+int launch_stage_quads(const void* proj, int dt, int nw, int nh, int nv, const float* norm,
+                       void* quad, hipStream_t stream) {
+  const int QW = nw + 2, QH = nh + 2;
+  FC_CHECK_ARG(QH <= 65535 && nv <= 65535, "projection stack too large for the staging grid");
+  if (nv == 0) return FC_OK;
+  dim3 sg((QW + 255) / 256, QH, nv);
+  if (dt == FC_F32) stage_quads<FC_F32><<<sg, 256, 0, stream>>>(proj, nw, nh, norm, quad, QW, QH);
+  else if (dt == FC_BF16) stage_quads<FC_BF16><<<sg, 256, 0, stream>>>(proj, nw, nh, norm, quad, QW, QH);
+  else stage_quads<FC_F16><<<sg, 256, 0, stream>>>(proj, nw, nh, norm, quad, QW, QH);
+  FC_HIP(hipGetLastError());
+  return FC_OK;
+}
+
 int launch_gather_bench_bp(int device, int dt, double* samples_per_s) {
   (void)device;
   const int QW = 1024, QH = 1024;

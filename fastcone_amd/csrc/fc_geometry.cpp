@@ -108,35 +108,36 @@ extern "C" int fc_views_flat(int nviews, const double* angles, const double* hmi
   return FC_OK;
 }
 
-extern "C" int fc_views_cylinder(int nviews, const double* angles, const double* src_z, double R,
-                                 double SDD, double dgamma, double dv_row, const int proj_dim[2],
-                                 const int* woff, const int* hoff, double* out) {
+extern "C" int fc_views_cylinder(int nviews, const double* lambdas, const double* src_z,
+                                 double R, double SDD, double dgamma, double drow,
+                                 const double* col_c, const double* row_c, double* out) {
   fc::clear_error();
   FC_CHECK_ARG(nviews >= 0, "nviews must be >= 0 (got %d)", nviews);
-  FC_CHECK_ARG(nviews == 0 || (angles && src_z && out && proj_dim), "null geometry pointer");
+  FC_CHECK_ARG(nviews == 0 || (lambdas && src_z && out), "null geometry pointer");
   FC_CHECK_ARG(std::isfinite(R) && R > 0 && std::isfinite(SDD) && SDD > R,
                "need 0 < R < SDD (got R=%g SDD=%g)", R, SDD);
-  FC_CHECK_ARG(std::isfinite(dgamma) && dgamma > 0 && std::isfinite(dv_row) && dv_row > 0,
+  FC_CHECK_ARG(std::isfinite(dgamma) && dgamma > 0 && std::isfinite(drow) && drow > 0,
                "detector pitches must be > 0");
-  const int Nw = proj_dim ? proj_dim[0] : 0, Nh = proj_dim ? proj_dim[1] : 0;
   for (int t = 0; t < nviews; ++t) {
-    const double a = angles[t];
-    FC_CHECK_ARG(std::isfinite(a) && std::isfinite(src_z[t]), "view %d: non-finite", t);
-    V3 s = {-std::cos(a) * R, -std::sin(a) * R, src_z[t]};
-    V3 ec = {std::cos(a), std::sin(a), 0.0};    // source -> axis
-    V3 ew = {-std::sin(a), std::cos(a), 0.0};   // tangential, same sense as the flat w axis
+    const double l = lambdas[t];
+    FC_CHECK_ARG(std::isfinite(l) && std::isfinite(src_z[t]), "view %d: non-finite", t);
+    // helix a(l) = (R cos l, R sin l, z); e_v = -(cos l, sin l, 0) points at the axis,
+    // e_u = (-sin l, cos l, 0); a column at fan angle g looks along cos g e_v + sin g e_u
+    V3 s = {R * std::cos(l), R * std::sin(l), src_z[t]};
+    V3 ev = {-std::cos(l), -std::sin(l), 0.0};
+    V3 eu = {-std::sin(l), std::cos(l), 0.0};
     double* rec = out + (size_t)t * FC_VIEW_DOUBLES;
     std::memset(rec, 0, sizeof(double) * FC_VIEW_DOUBLES);
     put(rec + 0, s);
-    put(rec + 3, SDD * ec);
-    put(rec + 6, SDD * ew);
-    put(rec + 9, V3{0.0, 0.0, dv_row});
+    put(rec + 3, SDD * ev);
+    put(rec + 6, SDD * eu);
+    put(rec + 9, V3{0.0, 0.0, drow});
     rec[24] = (double)FC_DET_CYLINDER;
-    // window column j has fan angle (j - rec[25]) * dgamma; row i height (i - rec[27]) * dv_row
-    rec[25] = (double)(Nw / 2 - (woff ? woff[t] : 0));
+    rec[25] = col_c ? col_c[t] : 0.0;
     rec[26] = dgamma;
-    rec[27] = (double)(Nh / 2 - (hoff ? hoff[t] : 0));
+    rec[27] = row_c ? row_c[t] : 0.0;
     rec[28] = SDD;
+    FC_CHECK_ARG(std::isfinite(rec[25]) && std::isfinite(rec[27]), "view %d: bad centre", t);
   }
   return FC_OK;
 }

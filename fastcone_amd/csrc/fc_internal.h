@@ -44,7 +44,8 @@ int select_device(int device);
 // A call that uses a workspace on a stream other than the one its previous user ran on first
 // makes the new stream wait on an event recorded after that previous use, so reuse is ordered.
 enum Slot { SLOT_OCTET = 0, SLOT_QUAD = 1, SLOT_H2D_VOL = 2, SLOT_H2D_PROJ = 3,
-            SLOT_H2D_VIEWS = 4, SLOT_H2D_NORM = 5, SLOT_COUNT = 6 };
+            SLOT_H2D_VIEWS = 4, SLOT_H2D_NORM = 5, SLOT_KAT_A = 6, SLOT_KAT_B = 7,
+            SLOT_KAT_TAB = 8, SLOT_COUNT = 9 };
 
 struct Workspace {
   void* ptr = nullptr;
@@ -83,6 +84,9 @@ int launch_backproject(int device, float* vol, const int vol_shape[3], const dou
 int launch_sample_count(int device, const int vol_shape[3], const double origin[3],
                         const int proj_shape[3], const double* views, double step,
                         long long* counts, hipStream_t stream);
+// Stage projections into the backprojector's quad layout (norm may be NULL).
+int launch_stage_quads(const void* proj, int dt, int nw, int nh, int nv, const float* norm,
+                       void* quad, hipStream_t stream);
 int launch_gather_bench_fp(int device, int dt, int plain, double* samples_per_s);
 int launch_gather_bench_bp(int device, int dt, double* samples_per_s);
 int launch_convert(const void* src, int src_dtype, void* dst, int dst_dtype, long long n,

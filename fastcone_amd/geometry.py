@@ -84,30 +84,38 @@ class Geometry:
         return cls(views, vol_shape, origin, det_shape, "flat")
 
     @classmethod
-    def cylinder(cls, angles, src_z, R, SDD, dgamma, dv_row, vol_shape, det_shape, vol_dim=None,
-                 vol_start=(0, 0, 0), woff=None, hoff=None, det_dim=None):
-        """Curved (cylindrical, source-centred) detector; src_z is the helical source height
-        (coordinate, voxels) per view; dgamma the column pitch in radians; dv_row the row pitch
-        at the detector (voxels)."""
-        angles = np.ascontiguousarray(angles, dtype=np.float64).reshape(-1)
-        n = angles.size
+    def cylinder(cls, lambdas, src_z, R, SDD, dgamma, drow, vol_shape, det_shape,
+                 col_c=None, row_c=None, vol_dim=None, vol_start=(0, 0, 0)):
+        """Curved (cylindrical, source-centred) detector on a circular or helical source path
+        a(l) = (R cos l, R sin l, src_z) in voxels. Column j has fan angle (j - col_c) * dgamma
+        (radians), row i height (i - row_c) * drow at the detector. col_c / row_c default to the
+        symmetric centres (nw - 1) / 2 and (nh - 1) / 2."""
+        lambdas = np.ascontiguousarray(lambdas, dtype=np.float64).reshape(-1)
+        n = lambdas.size
+        vol_shape = tuple(int(v) for v in vol_shape)
         vol_dim = tuple(int(v) for v in (vol_dim if vol_dim is not None else vol_shape))
         det_shape = tuple(int(v) for v in det_shape)
-        det_dim = tuple(int(v) for v in (det_dim if det_dim is not None else det_shape))
+        nh, nw = det_shape
+        col_c = np.full(n, (nw - 1) / 2.0) if col_c is None else np.broadcast_to(
+            np.asarray(col_c, np.float64), (n,))
+        row_c = np.full(n, (nh - 1) / 2.0) if row_c is None else np.broadcast_to(
+            np.asarray(row_c, np.float64), (n,))
+        col_c = np.ascontiguousarray(col_c, dtype=np.float64)
+        row_c = np.ascontiguousarray(row_c, dtype=np.float64)
         src_z = np.ascontiguousarray(np.broadcast_to(np.asarray(src_z, np.float64), (n,)))
-        woff = np.zeros(n, np.int32) if woff is None else np.ascontiguousarray(woff, np.int32)
-        hoff = np.zeros(n, np.int32) if hoff is None else np.ascontiguousarray(hoff, np.int32)
         views = np.zeros((n, _native.FC_VIEW_DOUBLES), np.float64)
         L = _native.lib()
-        pd = _native.int_array([det_dim[1], det_dim[0]])
         rc = L.fc_views_cylinder(
-            n, angles.ctypes.data_as(_native._c_dbl_p), src_z.ctypes.data_as(_native._c_dbl_p),
-            float(R), float(SDD), float(dgamma), float(dv_row), pd,
-            woff.ctypes.data_as(_native._c_int_p), hoff.ctypes.data_as(_native._c_int_p),
+            n, lambdas.ctypes.data_as(_native._c_dbl_p), src_z.ctypes.data_as(_native._c_dbl_p),
+            float(R), float(SDD), float(dgamma), float(drow),
+            col_c.ctypes.data_as(_native._c_dbl_p), row_c.ctypes.data_as(_native._c_dbl_p),
             views.ctypes.data_as(_native._c_dbl_p))
         _native.check(rc)
         vs = np.asarray(vol_start, dtype=np.int64)
         vd = np.asarray(vol_dim, dtype=np.int64)
         origin = np.array([vs[2] - vd[2] // 2, vs[1] - vd[1] // 2, vs[0] - vd[0] // 2],
                           dtype=np.float64)
-        return cls(views, vol_shape, origin, det_shape, "cylinder")
+        g = cls(views, vol_shape, origin, det_shape, "cylinder")
+        g.helix = dict(lambdas=lambdas, src_z=src_z, R=float(R), SDD=float(SDD),
+                       dgamma=float(dgamma), drow=float(drow), col_c=col_c, row_c=row_c)
+        return g

@@ -116,13 +116,15 @@ int fc_views_flat(int nviews, const double* angles, const double* hmid, double D
                   double pw_h, const double* mis, int vol_dim_z, const int proj_dim[2],
                   const int* woff, const int* hoff, double* views_out);
 
-/* Cylindrical (curved) detector centred on the source: column j sits at fan angle
- * (j + woff - Nw/2) * dgamma, row i at height (i + hoff - Nh/2) * dv_row measured at the detector
- * (voxels). src_z (voxels, coordinate) gives the helical source height per view; R source-to-axis,
- * SDD source-to-detector (voxels). */
-int fc_views_cylinder(int nviews, const double* angles, const double* src_z, double R, double SDD,
-                      double dgamma, double dv_row, const int proj_dim[2], const int* woff,
-                      const int* hoff, double* views_out);
+/* Cylindrical (curved) detector centred on the source, helical or circular source path
+ * a(l) = (R cos l, R sin l, src_z) (voxels). Window column j looks along the fan angle
+ * g = (j - col_c[t]) * dgamma, measured from e_v = -(cos l, sin l, 0) towards
+ * e_u = (-sin l, cos l, 0); row i sits at height (i - row_c[t]) * drow on the detector (voxels),
+ * SDD from the source. col_c / row_c may be NULL (= 0). No reference counterpart (the reference
+ * represents only a flat detector with one pitch, fastcone.py:72). */
+int fc_views_cylinder(int nviews, const double* lambdas, const double* src_z, double R,
+                      double SDD, double dgamma, double drow, const double* col_c,
+                      const double* row_c, double* views_out);
 
 /* Forward projection on the device:
  *   proj[t][i][j] += norm[t] * step * sum_k T(src_t + k*step*u_tij)
@@ -146,6 +148,20 @@ int fc_adjoint_dev(int device, float* vol, const int vol_shape[3], const double 
 int fc_backproject_dev(int device, float* vol, const int vol_shape[3], const double origin[3],
                        const void* proj, int proj_dtype, const int proj_shape[3],
                        const double* views, const float* norm, int mode, void* stream);
+
+/* Katsevich exact helical reconstruction on a cylindrical detector (no reference counterpart: the
+ * reference offers only a U-weighted backprojection of pre-filtered data, GPU_CB_K_backproject
+ * mode 1, coneBeamGPU.cu:258-338). proj: device [nviews][nh][nw] line integrals (dtype fc_dtype)
+ * taken at helix angles l_k = l0 + k dl with source height zs_k = zs0 + k dl hp (voxels);
+ * column j at fan angle (j - col_c) dgamma, row i at height (i - row_c) drow at distance SDD from
+ * the source (see fc_views_cylinder). npsi kappa lines. vol (device FP32, [nz][ny][nx], voxel 0
+ * at coordinate origin[]) is accumulated. g5_out (device, nviews*nh*nw FP32) may receive the
+ * filtered projections (NULL to skip). */
+int fc_katsevich_dev(int device, const void* proj, int proj_dtype, int nw, int nh, int nviews,
+                     double l0, double dl, double zs0, double hp, double R, double SDD,
+                     double dgamma, double drow, double col_c, double row_c, int npsi,
+                     float* vol, const int vol_shape[3], const double origin[3], float* g5_out,
+                     void* stream);
 
 /* Exact sample count of fc_project_dev per view (int64 device array counts[nviews], overwritten). */
 int fc_sample_count_dev(int device, const int vol_shape[3], const double origin[3],

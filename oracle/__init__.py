@@ -336,3 +336,75 @@ def ellipsoid_line_integrals(src, pts, ellipsoids):
         disc = qa * qa - aa * cc
         out += mu * np.where(disc > 0, 2.0 * np.sqrt(np.maximum(disc, 0.0)) / aa, 0.0)
     return out
+
+
+# ----------------------------------------------------------------------------------------------
+# Helical cone beam, cylindrical detector (Katsevich)
+# ----------------------------------------------------------------------------------------------
+class Helix:
+    """a(l) = (R cos l, R sin l, zs(l)) in voxels; column j at fan angle (j - col_c) dg from
+    e_v = -(cos l, sin l, 0) towards e_u = (-sin l, cos l, 0); row i at height (i - row_c) drow on
+    the detector at distance SDD. hp = dzs/dl (voxels per radian), dl = view spacing."""
+
+    def __init__(self, lambdas, zs, R, SDD, dg, drow, nw, nh, hp, col_c=None, row_c=None):
+        self.lambdas = np.ascontiguousarray(lambdas, np.float64)
+        self.zs = np.ascontiguousarray(np.broadcast_to(np.asarray(zs, np.float64),
+                                                       self.lambdas.shape))
+        self.R, self.SDD, self.dg, self.drow = float(R), float(SDD), float(dg), float(drow)
+        self.nw, self.nh = int(nw), int(nh)
+        self.hp = float(hp)
+        self.col_c = (nw - 1) / 2.0 if col_c is None else float(col_c)
+        self.row_c = (nh - 1) / 2.0 if row_c is None else float(row_c)
+        self.dl = float(self.lambdas[1] - self.lambdas[0]) if self.lambdas.size > 1 else 1.0
+
+    @property
+    def n(self):
+        return self.lambdas.size
+
+    def ray_points(self, t):
+        """Source a(l_t) [3] and one point on each pixel's ray [nh, nw, 3]."""
+        l = self.lambdas[t]
+        a = np.array([self.R * np.cos(l), self.R * np.sin(l), self.zs[t]])
+        ev = np.array([-np.cos(l), -np.sin(l), 0.0])
+        eu = np.array([-np.sin(l), np.cos(l), 0.0])
+        g = (np.arange(self.nw) - self.col_c) * self.dg
+        w = (np.arange(self.nh) - self.row_c) * self.drow
+        d = (self.SDD * np.cos(g)[None, :, None] * ev + self.SDD * np.sin(g)[None, :, None] * eu
+             + w[:, None, None] * np.array([0.0, 0.0, 1.0]))
+        return a, a + d
+
+
+def project_helical(hx, vol, origin, h=0.5):
+    vol, pv = _d(vol)
+    nz, ny, nx = vol.shape
+    proj = np.zeros((hx.n, hx.nh, hx.nw))
+    org, po = _d(origin)
+    lam, pl = _d(hx.lambdas)
+    zs, pz = _d(hx.zs)
+    D = ctypes.c_double
+    lib().orc_project_helical(pv, nx, ny, nz, po, proj.ctypes.data_as(_dp), hx.nw, hx.nh, hx.n,
+                              pl, pz, D(hx.R), D(hx.SDD), D(hx.dg), D(hx.drow), D(hx.col_c),
+                              D(hx.row_c), D(h))
+    return proj
+
+
+KATSEVICH_SIGN = 1.0
+
+
+def katsevich(hx, proj, vol_shape, origin, npsi=None, sign=None, return_g5=False):
+    """FP64 Katsevich reconstruction (see oracle.cpp: orc_katsevich)."""
+    proj, pp = _d(proj)
+    nz, ny, nx = vol_shape
+    vol = np.zeros(vol_shape)
+    npsi = 2 * hx.nh + 1 if npsi is None else int(npsi)
+    org, po = _d(origin)
+    lam, pl = _d(hx.lambdas)
+    zs, pz = _d(hx.zs)
+    g5 = np.zeros(proj.shape) if return_g5 else None
+    D = ctypes.c_double
+    lib().orc_katsevich(pp, hx.nw, hx.nh, hx.n, pl, pz, D(hx.R), D(hx.SDD), D(hx.dg),
+                        D(hx.drow), D(hx.col_c), D(hx.row_c), D(hx.hp), D(hx.dl),
+                        ctypes.c_int(npsi), D(KATSEVICH_SIGN if sign is None else sign),
+                        vol.ctypes.data_as(_dp), nx, ny, nz, po,
+                        g5.ctypes.data_as(_dp) if return_g5 else None)
+    return (vol, g5) if return_g5 else vol

@@ -365,6 +365,176 @@ void orc_backproject_general(double* vol, int nx, int ny, int nz, const double* 
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Helical cone beam on a cylindrical detector centred on the source (no reference counterpart:
+// the reference ABI represents only a flat detector with one pitch, fastcone.py:72, and applies
+// no Katsevich filtering, coneBeamGPU.cu:258-338). Helix a(l) = (R cos l, R sin l, zs);
+// e_v = -(cos l, sin l, 0), e_u = (-sin l, cos l, 0); column j looks along fan angle
+// g = (j - col_c) * dg from e_v towards e_u; row i is at height w = (i - row_c) * drow on the
+// detector at distance SDD from the source.
+// ---------------------------------------------------------------------------------------------
+
+// Forward projection h * sum_k T(a + k h u) along the helical curved-detector rays.
+void orc_project_helical(const double* vol, int nx, int ny, int nz, const double* origin,
+                         double* proj, int nw, int nh, int nviews, const double* lambdas,
+                         const double* zs, double R, double SDD, double dg, double drow,
+                         double col_c, double row_c, double h) {
+  const int n[3] = {nx, ny, nz};
+  const V3 o = load3(origin);
+#pragma omp parallel for collapse(2) schedule(dynamic, 4)
+  for (int t = 0; t < nviews; ++t) {
+    for (int i = 0; i < nh; ++i) {
+      const double l = lambdas[t];
+      V3 a = {R * std::cos(l), R * std::sin(l), zs[t]};
+      V3 ev = {-std::cos(l), -std::sin(l), 0.0}, eu = {-std::sin(l), std::cos(l), 0.0};
+      V3 al = a - o;
+      for (int j = 0; j < nw; ++j) {
+        double g = (j - col_c) * dg, w = (i - row_c) * drow;
+        V3 d = (SDD * std::cos(g)) * ev + (SDD * std::sin(g)) * eu + V3{0.0, 0.0, w};
+        V3 u = normalize(d);
+        long k0, k1;
+        double sum = 0.0;
+        if (ray_k_range(al, u, n, h, &k0, &k1)) {
+          for (long k = k0; k <= k1; ++k) {
+            double kh = (double)k * h;
+            sum += trilinear(vol, nx, ny, nz, al.x + kh * u.x, al.y + kh * u.y, al.z + kh * u.z);
+          }
+        }
+        proj[((long)t * nh + i) * nw + j] = h * sum;
+      }
+    }
+  }
+}
+
+// kappa-line height on the detector: w_k(g, psi) = (SDD hp / R) (psi cos g + psi cot psi sin g)
+static inline double kappa_w(double g, double psi, double SDD, double hp, double R) {
+  double pc = std::fabs(psi) < 1e-12 ? 1.0 : psi / std::tan(psi);
+  return SDD * hp / R * (psi * std::cos(g) + pc * std::sin(g));
+}
+
+// Katsevich reconstruction (Katsevich 2002; native curved-detector form of Noo, Pack & Heuscher
+// 2003, Phys. Med. Biol. 48:3787), discretised as:
+//   g1 = [G(k+1, g + dl) - G(k-1, g - dl)] / (2 dl)     derivative at a fixed ray direction
+//        (G linear in the column, zero outside; g1 = 0 for the first and last view)
+//   g2 = g1 * SDD / sqrt(SDD^2 + w^2)                    cone-angle weight
+//   g3(psi_m) = g2 at w = w_k(g, psi_m), linear in the row   (forward height rebinning)
+//   g4 = sum_{n odd} 2/(pi n) * (n dg / sin(n dg)) g3[j-n]     Hilbert kernel 1/(pi sin)
+//   g5(w) = g4 at psi with w_k(g, psi) = w, linear between psi_m   (backward rebinning)
+//   f(x) = sign/(2 pi) * sum_k dl * g5(k, g*, w*) / v*  over views whose projection of x lies in
+//          the Tam-Danielsson window  -(SDD hp/R)(pi/2+g)/cos g <= w <= (SDD hp/R)(pi/2-g)/cos g
+// with v* = R - x cos l - y sin l, g* = atan(u*/v*), w* = SDD (z - zs) cos g* / v*.
+// psi_m uniform on [-(pi/2 + am), pi/2 + am], am = the largest |fan angle| of the detector.
+void orc_katsevich(const double* proj, int nw, int nh, int nviews, const double* lambdas,
+                   const double* zs, double R, double SDD, double dg, double drow, double col_c,
+                   double row_c, double hp, double dl, int npsi, double sign, double* vol,
+                   int nx, int ny, int nz, const double* origin, double* g5_out) {
+  const long plane = (long)nh * nw;
+  std::vector<double> g2((size_t)nviews * plane, 0.0);
+  auto G = [&](int k, int i, double c) {  // linear in the column, zero-extended
+    double fc = std::floor(c);
+    long j0 = (long)fc;
+    double t = c - fc, s = 0.0;
+    const double* row = proj + ((long)k * nh + i) * nw;
+    if (j0 >= 0 && j0 < nw) s += (1.0 - t) * row[j0];
+    if (j0 + 1 >= 0 && j0 + 1 < nw) s += t * row[j0 + 1];
+    return s;
+  };
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 1; k < nviews - 1; ++k) {
+    for (int i = 0; i < nh; ++i) {
+      double w = (i - row_c) * drow;
+      double cw = SDD / std::sqrt(SDD * SDD + w * w);
+      for (int j = 0; j < nw; ++j) {
+        double shift = dl / dg;
+        double d = (G(k + 1, i, j + shift) - G(k - 1, i, j - shift)) / (2.0 * dl);
+        g2[(size_t)k * plane + (long)i * nw + j] = d * cw;
+      }
+    }
+  }
+  double am = std::max(std::fabs(0 - col_c), std::fabs(nw - 1 - col_c)) * dg;
+  double pmax = M_PI / 2 + am;
+  double dpsi = 2 * pmax / (npsi - 1);
+  // kappa-line tables per column
+  std::vector<double> W((size_t)nw * npsi);
+  for (int j = 0; j < nw; ++j)
+    for (int m = 0; m < npsi; ++m)
+      W[(size_t)j * npsi + m] = kappa_w((j - col_c) * dg, -pmax + m * dpsi, SDD, hp, R);
+  // Hilbert kernel
+  std::vector<double> hk(2 * nw - 1, 0.0);
+  for (int n = -(nw - 1); n <= nw - 1; ++n) {
+    if (n % 2 == 0) continue;
+    double x = n * dg;
+    hk[n + nw - 1] = 2.0 / (M_PI * n) * (x / std::sin(x));
+  }
+  std::vector<double> g5((size_t)nviews * plane, 0.0);
+#pragma omp parallel
+  {
+    std::vector<double> g3((size_t)npsi * nw), g4((size_t)npsi * nw);
+#pragma omp for schedule(static)
+    for (int k = 1; k < nviews - 1; ++k) {
+      const double* G2 = g2.data() + (size_t)k * plane;
+      for (int m = 0; m < npsi; ++m)
+        for (int j = 0; j < nw; ++j) {
+          double r = W[(size_t)j * npsi + m] / drow + row_c;
+          double fr = std::floor(r);
+          long i0 = (long)fr;
+          double t = r - fr, v = 0.0;
+          if (i0 >= 0 && i0 < nh) v += (1 - t) * G2[i0 * nw + j];
+          if (i0 + 1 >= 0 && i0 + 1 < nh) v += t * G2[(i0 + 1) * nw + j];
+          g3[(size_t)m * nw + j] = v;
+        }
+      for (int m = 0; m < npsi; ++m)
+        for (int j = 0; j < nw; ++j) {
+          double acc = 0.0;
+          for (int jj = 0; jj < nw; ++jj) acc += hk[j - jj + nw - 1] * g3[(size_t)m * nw + jj];
+          g4[(size_t)m * nw + j] = acc;
+        }
+      for (int i = 0; i < nh; ++i) {
+        double w = (i - row_c) * drow;
+        for (int j = 0; j < nw; ++j) {
+          const double* Wj = W.data() + (size_t)j * npsi;
+          double v = 0.0;
+          if (w >= Wj[0] && w <= Wj[npsi - 1]) {
+            int lo = 0, hi = npsi - 1;
+            while (hi - lo > 1) {
+              int mid = (lo + hi) / 2;
+              if (Wj[mid] <= w) lo = mid; else hi = mid;
+            }
+            double t = (w - Wj[lo]) / (Wj[hi] - Wj[lo]);
+            v = (1 - t) * g4[(size_t)lo * nw + j] + t * g4[(size_t)hi * nw + j];
+          }
+          g5[(size_t)k * plane + (long)i * nw + j] = v;
+        }
+      }
+    }
+  }
+  if (g5_out) std::memcpy(g5_out, g5.data(), sizeof(double) * g5.size());
+  const double c0 = SDD * hp / R;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int iz = 0; iz < nz; ++iz) {
+    for (int iy = 0; iy < ny; ++iy) {
+      double z = iz + origin[2], y = iy + origin[1];
+      for (int ix = 0; ix < nx; ++ix) {
+        double x = ix + origin[0];
+        double acc = 0.0;
+        for (int k = 0; k < nviews; ++k) {
+          double l = lambdas[k];
+          double vs = R - x * std::cos(l) - y * std::sin(l);
+          double us = -x * std::sin(l) + y * std::cos(l);
+          double g = std::atan(us / vs);
+          double cg = std::cos(g);
+          double w = SDD * (z - zs[k]) * cg / vs;
+          double wt = c0 * (M_PI / 2 - g) / cg, wb = -c0 * (M_PI / 2 + g) / cg;
+          if (w < wb || w > wt) continue;
+          double c = g / dg + col_c, r = w / drow + row_c;
+          acc += bilinear(g5.data() + (size_t)k * plane, nw, nh, c, r) / vs;
+        }
+        vol[((long)iz * ny + iy) * nx + ix] += sign * dl / (2 * M_PI) * acc;
+      }
+    }
+  }
+}
+
 int orc_num_threads(void) { return omp_get_max_threads(); }
 
 }  // extern "C"
