@@ -163,6 +163,25 @@ int fc_katsevich_dev(int device, const void* proj, int proj_dtype, int nw, int n
                      float* vol, const int vol_shape[3], const double origin[3], float* g5_out,
                      void* stream);
 
+/* 3x3x3 convolution (zero padding 1) for the CGLS warm-start network; no reference counterpart.
+ * Activations N x D x H x W x C (channels last) BF16 on the device, weights [cout][27][cin] BF16
+ * (tap = 9 dz + 3 dy + dx), bias FP32, FP32 accumulation (MFMA). Channel pairs: the network's
+ * layers (1->16, 16->16, 16->32, 32->32, 32->64, 64->64, 96->32, 48->16, 16->1) and, for the
+ * input gradient, their transposes. The cout = 1 layer writes / reads FP32. */
+int fc_conv3d_fwd_dev(int device, const void* x, const void* w, const float* bias, void* y,
+                      int N, int D, int H, int W, int cin, int cout, int relu, void* stream);
+/* dx = input gradient of the convolution for output gradient dy; wt is a device workspace of
+ * cin*27*cout BF16 elements (receives the flipped, transposed weights). */
+int fc_conv3d_dgrad_dev(int device, const void* dy, const void* w, void* wt, void* dx, int N,
+                        int D, int H, int W, int cin, int cout, void* stream);
+/* dw (FP32 [cout][27][cin]) += weight gradient for input x and output gradient dy. */
+int fc_conv3d_wgrad_dev(int device, const void* x, const void* dy, float* dw, int N, int D, int H,
+                        int W, int cin, int cout, void* stream);
+/* One 16x16x32 BF16 and one 16x16x4 FP32 MFMA product of row-major A, B (single wave): checks the
+ * operand / result lane layouts the convolution kernels assume. */
+int fc_mfma_layout_probe(int device, const void* A32, const void* B32, const float* A4,
+                         const float* B4, float* D32, float* D4, void* stream);
+
 /* Exact sample count of fc_project_dev per view (int64 device array counts[nviews], overwritten). */
 int fc_sample_count_dev(int device, const int vol_shape[3], const double origin[3],
                         const int proj_shape[3], const double* views, double step,

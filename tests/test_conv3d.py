@@ -1,0 +1,105 @@
+"""Hand-written conv3d kernels (MFMA) against torch.nn.functional.conv3d in FP32 on the CPU, fed
+the same BF16-rounded inputs and weights. Tolerance: FP32 accumulation in a different order, and
+BF16 rounding of the outputs (relative 2^-8), so rel. L2 < 1e-2 (outputs) and < 2e-3 (FP32
+weight gradients)."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+PAIRS = [(16, 16), (16, 32), (32, 32), (32, 64), (64, 64), (96, 32), (48, 16)]
+
+
+def rel(a, b):
+    a = a.double()
+    b = b.double()
+    return float((a - b).norm() / b.norm())
+
+
+def _torch_w(w):  # [co][27][ci] -> [co][ci][3][3][3]
+    co, _, ci = w.shape
+    return w.reshape(co, 3, 3, 3, ci).permute(0, 4, 1, 2, 3).contiguous()
+
+
+def test_mfma_lane_layout(gpu):
+    from fastcone_amd import _native
+    import ctypes
+    g = torch.Generator().manual_seed(0)
+    A32 = torch.randn(16, 32, generator=g).to(torch.bfloat16)
+    B32 = torch.randn(32, 16, generator=g).to(torch.bfloat16)
+    A4 = torch.randn(16, 4, generator=g)
+    B4 = torch.randn(4, 16, generator=g)
+    d = {k: v.to(gpu).contiguous() for k, v in dict(A32=A32, B32=B32, A4=A4, B4=B4).items()}
+    D32 = torch.zeros(16, 16, device=gpu)
+    D4 = torch.zeros(16, 16, device=gpu)
+    rc = _native.lib().fc_mfma_layout_probe(0, d["A32"].data_ptr(), d["B32"].data_ptr(),
+                                            d["A4"].data_ptr(), d["B4"].data_ptr(),
+                                            D32.data_ptr(), D4.data_ptr(),
+                                            ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    _native.check(rc)
+    torch.cuda.synchronize()
+    assert torch.allclose(D32.cpu(), A32.float() @ B32.float(), rtol=1e-5, atol=1e-4)
+    assert torch.allclose(D4.cpu(), A4 @ B4, rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.parametrize("cin,cout", PAIRS + [(1, 16), (16, 1)])
+def test_conv_forward_backward(gpu, cin, cout):
+    from fastcone_amd.unet import conv3d
+    g = torch.Generator().manual_seed(cin * 100 + cout)
+    N, D, H, W = 2, 6, 7, 20  # W not a multiple of 16: partial waves
+    x = torch.randn((N, D, H, W, cin), generator=g).to(torch.bfloat16)
+    w = (torch.randn((cout, 27, cin), generator=g) / np.sqrt(27 * cin))
+    b = torch.randn(cout, generator=g) * 0.1
+    relu = cout != 1
+    xg = x.to(gpu).requires_grad_(cin != 1)
+    wg = w.to(gpu).requires_grad_(True)
+    bg = b.to(gpu).requires_grad_(True)
+    y = conv3d(xg, wg, bg, relu)
+    # reference: FP32 CPU with the BF16-rounded weights the kernel reads
+    xr = x.float().permute(0, 4, 1, 2, 3).requires_grad_(True)
+    wr = _torch_w(w.to(torch.bfloat16).float()).requires_grad_(True)
+    br = b.clone().requires_grad_(True)
+    yr = torch.nn.functional.conv3d(xr, wr, br, padding=1)
+    if relu:
+        yr = torch.relu(yr)
+    yr_c = yr.permute(0, 2, 3, 4, 1)
+    assert rel(y.float().cpu(), yr_c) < 1e-2
+    gy = torch.randn(yr_c.shape, generator=g)
+    # the kernel's output gradient is taken in BF16 (or FP32 for the 1-channel output)
+    gy_k = gy.to(torch.bfloat16).float() if cout != 1 else gy
+    (y.float() * gy_k.to(gpu)).sum().backward()
+    (yr_c * gy_k).sum().backward()
+    # weight gradient from BF16 inputs: compare against the reference with the same rounding
+    assert rel(wg.grad.cpu(), wr.grad.permute(0, 2, 3, 4, 1).reshape(cout, 27, cin)) < 2e-2
+    assert rel(bg.grad.cpu(), br.grad) < 2e-2
+    if cin != 1:
+        assert rel(xg.grad.float().cpu(), xr.grad.permute(0, 2, 3, 4, 1)) < 2e-2
+
+
+def test_unet_matches_fp32_reference(gpu):
+    from fastcone_amd.unet import UNet3D, LAYERS
+    net = UNet3D(seed=1).to(gpu)
+    g = torch.Generator().manual_seed(2)
+    vol = (torch.rand((2, 16, 16, 16), generator=g) * 0.02).to(gpu)
+    out = net(vol)
+    # FP32 reference with the same BF16-rounded weights
+    W = {k: _torch_w(v.detach().to(torch.bfloat16).float().cpu()) for k, v in net.w.items()}
+    B = {k: v.detach().float().cpu() for k, v in net.b.items()}
+    F = torch.nn.functional
+
+    def c(x, n, relu=True):
+        y = F.conv3d(x, W[n], B[n], padding=1)
+        return torch.relu(y) if relu else y
+
+    x = (vol.cpu() * UNet3D.SCALE)[:, None]
+    e1 = c(c(x, "e1a"), "e1b")
+    e2 = c(c(F.avg_pool3d(e1, 2), "e2a"), "e2b")
+    bt = c(c(F.avg_pool3d(e2, 2), "b_a"), "b_b")
+    up = lambda t: t.repeat_interleave(2, 2).repeat_interleave(2, 3).repeat_interleave(2, 4)
+    d2 = c(c(torch.cat([up(bt), e2], 1), "d2a"), "d2b")
+    d1 = c(c(torch.cat([up(d2), e1], 1), "d1a"), "d1b")
+    ref = vol.cpu() + c(d1, "out", relu=False)[:, 0] / UNet3D.SCALE
+    corr = out.detach().cpu() - vol.cpu()
+    corr_ref = ref - vol.cpu()
+    assert rel(corr, corr_ref) < 5e-2
