@@ -41,6 +41,14 @@ def parse():
     ap.add_argument("--no-variants", action="store_true", help="skip BF16/FP16 storage runs")
     ap.add_argument("--cpu-views", type=int, default=2)
     ap.add_argument("--cpu-slices", type=int, default=4)
+    ap.add_argument("--views", type=int, default=NV,
+                    help="views of the workload (default 720; fewer only for profiling runs)")
+    ap.add_argument("--random-volume", action="store_true",
+                    help="uniform random volume instead of the phantom (profiling runs)")
+    ap.add_argument("--cgls-model", default=os.path.join(ROOT, "fastcone_amd", "models",
+                                                         "unet_warmstart.npz"),
+                    help="trained warm-start network for the CGLS time-to-residual leg")
+    ap.add_argument("--no-cgls", action="store_true", help="skip the CGLS leg")
     return ap.parse_args()
 
 
@@ -61,15 +69,21 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     t_setup = time.time()
-    vol, _ = phantom.config_phantom(1, device=dev)
-    ang = 2 * np.pi * np.arange(NV) / NV
+    nv = args.views
+    if args.random_volume:
+        g0 = torch.Generator(device=dev)
+        g0.manual_seed(0)
+        vol = torch.rand((N, N, N), generator=g0, device=dev) * 0.02
+    else:
+        vol, _ = phantom.config_phantom(1, device=dev)
+    ang = 2 * np.pi * np.arange(nv) / nv
     geom = Geometry.circular(ang, D, PW, (N, N, N), (NDET, NDET))
     weights = torch.from_numpy(ops.fdk_cosine_weights(geom, D, PW)).to(dev, torch.float32)
-    norm = torch.full((NV,), 1.0 / NV, dtype=torch.float32, device=dev)
+    norm = torch.full((nv,), 1.0 / nv, dtype=torch.float32, device=dev)
     proj = torch.zeros(geom.proj_shape, dtype=torch.float32, device=dev)
     rec = torch.zeros(geom.vol_shape, dtype=torch.float32, device=dev)
     samples = int(ops.sample_count(geom, H, dev).sum().item())
-    W = float(N) ** 3 * NV
+    W = float(N) ** 3 * nv
     log(f"setup {time.time() - t_setup:.1f}s; FP samples {samples:.4e}")
 
     def step(v, store=torch.float32):
@@ -106,17 +120,17 @@ def main():
         f"BP kernel {bp_ms:.1f} ms")
 
     # gather roofline of the dominant kernel (the projector: 8 taps per sample)
-    peak_fp = _native.gather_bench(0, _native.FC_F32)
+    peak_fp = _native.gather_bench(3, _native.FC_F32)   # LDS-brick trilinear, as fp_lds_kernel
     peak_bp = _native.gather_bench(2, _native.FC_F32)
     fp_taps = 8.0 * samples
     bp_taps = 4.0 * W
     fp_rate = fp_taps / (fp_ms * 1e-3)
-    # compulsory HBM bytes of one projector launch: read the staged octet volume once, read and
-    # write the projection stack once
-    fp_hbm_bytes = (N + 3) ** 3 * 32 + 2 * 4 * NV * NDET * NDET
+    # compulsory HBM bytes of one projector launch: read the staged (padded) volume once, read
+    # and write the projection stack once
+    fp_hbm_bytes = (N + 4) ** 3 * 4 + 2 * 4 * nv * NDET * NDET
     roofline = {
         "bound": "gather",
-        "kernel": "fp_kernel (ray-march projector)",
+        "kernel": "fp_lds_kernel (ray-march projector, LDS slab bricks)",
         "achieved": fp_rate / 1e9, "peak": peak_fp * 8 / 1e9, "unit": "Gtap/s",
         "frac": fp_rate / (peak_fp * 8),
         "traffic": None,
@@ -151,8 +165,17 @@ def main():
 
     cpu = None
     parity = None
-    if not args.no_cpu:
+    if not args.no_cpu and nv == NV:
         cpu, parity = cpu_baseline(args, vol, geom, proj, rec, samples)
+
+    cgls_leg = None
+    if not args.no_cgls and nv == NV and os.path.exists(args.cgls_model):
+        del proj, rec
+        torch.cuda.empty_cache()
+        from fastcone_amd import warmstart
+        from fastcone_amd.unet import UNet3D
+        net = UNet3D().to(dev).load(args.cgls_model)
+        cgls_leg = warmstart.evaluate(net, dev, log=log)
 
     out = {
         "metric": METRIC, "value": W / wall / 1e9, "unit": "Gvoxel-view/s", "n_gpus": 1,
@@ -165,7 +188,7 @@ def main():
                    "pixel_pitch_vox": PW, "source_axis_vox": D, "step_vox": H,
                    "W_voxel_views": W, "fp_samples": samples},
         "kernels": kernels, "roofline": roofline, "variants": variants,
-        "cpu_baseline": cpu, "parity_on_cpu_sample": parity,
+        "cpu_baseline": cpu, "parity_on_cpu_sample": parity, "cgls_time_to_residual": cgls_leg,
     }
     print(json.dumps(out), flush=True)
 

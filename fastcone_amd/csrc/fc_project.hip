@@ -340,6 +340,26 @@ __device__ __forceinline__ uint32_t brick_elem_offset(const SlabPlan& P, const S
          dd * P.gsd + m1 * P.gs1 + m2 * P.gs2;
 }
 
+// Trilinear sample from an LDS brick laid out dominant axis fastest (pitch DP), then minor axis 1
+// (E1 rows), then minor axis 2 (plane of zbytes bytes). `base` is the brick address minus the
+// brick origin's offset, so the corner of a sample at brick-frame-relative coordinates
+// (qd, q1, q2) >= 0 is at index qd + DP (q1 + E1 q2). Truncation is floor for q >= 0; the 24-bit
+// multiply-adds keep the index math full rate.
+template <int DP>
+__device__ __forceinline__ float lds_trilinear(const char* base, uint32_t E1u, uint32_t zbytes,
+                                               float qd, float q1, float q2) {
+  const uint32_t id = (uint32_t)(int)qd, i1 = (uint32_t)(int)q1, i2 = (uint32_t)(int)q2;
+  const float ad = __builtin_amdgcn_fractf(qd), a1f = __builtin_amdgcn_fractf(q1),
+              a2f = __builtin_amdgcn_fractf(q2);
+  const uint32_t row = mad_u24(i2, E1u, i1);
+  const float* L0 = reinterpret_cast<const float*>(base + 4u * mad_u24(row, (uint32_t)DP, id));
+  const float* L1 = reinterpret_cast<const float*>(reinterpret_cast<const char*>(L0) + zbytes);
+  float c00 = lerp(L0[0], L0[1], ad), c10 = lerp(L0[DP], L0[DP + 1], ad);
+  float c01 = lerp(L1[0], L1[1], ad), c11 = lerp(L1[DP], L1[DP + 1], ad);
+  float c0 = lerp(c00, c10, a1f), c1 = lerp(c01, c11, a1f);
+  return lerp(c0, c1, a2f);
+}
+
 template <int DT, int S>
 __global__ __launch_bounds__(256) void fp_lds_kernel(FpArgs a) {
   constexpr int DP = S + 3;  // dominant-axis pitch of the LDS brick
@@ -413,20 +433,8 @@ __global__ __launch_bounds__(256) void fp_lds_kernel(FpArgs a) {
       const uint32_t zbytes = 4u * (uint32_t)PS;  // one m2 plane
       const char* base = reinterpret_cast<const char*>(brick) - 4u * boff;
       auto sample = [&](float kf) {
-        const float qd = fmaf(kf, P.sd, P.pd0), q1 = fmaf(kf, P.s1, P.p10),
-                    q2 = fmaf(kf, P.s2, P.p20);
-        const uint32_t id = (uint32_t)(int)qd, i1 = (uint32_t)(int)q1, i2 = (uint32_t)(int)q2;
-        const float ad = __builtin_amdgcn_fractf(qd), a1f = __builtin_amdgcn_fractf(q1),
-                    a2f = __builtin_amdgcn_fractf(q2);
-        const uint32_t row = mad_u24(i2, E1u, i1);
-        const float* L0 =
-            reinterpret_cast<const float*>(base + 4u * mad_u24(row, (uint32_t)DP, id));
-        const float* L1 =
-            reinterpret_cast<const float*>(reinterpret_cast<const char*>(L0) + zbytes);
-        float c00 = lerp(L0[0], L0[1], ad), c10 = lerp(L0[DP], L0[DP + 1], ad);
-        float c01 = lerp(L1[0], L1[1], ad), c11 = lerp(L1[DP], L1[DP + 1], ad);
-        float c0 = lerp(c00, c10, a1f), c1 = lerp(c01, c11, a1f);
-        return lerp(c0, c1, a2f);
+        return lds_trilinear<DP>(base, E1u, zbytes, fmaf(kf, P.sd, P.pd0),
+                                 fmaf(kf, P.s1, P.p10), fmaf(kf, P.s2, P.p20));
       };
       // two samples per iteration: both samples' LDS reads are in flight together
       float kf = (float)B.kin;
@@ -678,6 +686,45 @@ __global__ __launch_bounds__(256) void gather_bench_fp(FpArgs a, int B, int nsam
   if (acc == 12345.678f) sink[0] = acc;  // keep the work alive
 }
 
+// LDS variant: each block stages one DP x E x E brick of the plain staged volume in LDS (same
+// layout as fp_lds_kernel), then draws trilinear samples at pseudo-random positions inside it
+// with the projector's own LDS fetch and interpolation (two samples per iteration).
+template <int S>
+__global__ __launch_bounds__(256) void gather_bench_lds(FpArgs a, int E, int nsamp,
+                                                        float* __restrict__ sink) {
+  constexpr int DP = S + 3;
+  __shared__ float brick[kLdsCap];
+  const int NE = DP * E * E;
+  const int nb = (a.nx - E) / E;
+  const int b = blockIdx.x % (nb * nb);
+  const uint32_t g0 = (uint32_t)(2 + (b % nb) * E) * (uint32_t)a.PX +
+                      (uint32_t)(2 + (b / nb) * E) * (uint32_t)a.PX * (uint32_t)a.PY + 2u;
+  for (int e = threadIdx.x; e < NE; e += 256) {
+    const int d = e % DP, rest = e / DP, m1 = rest % E, m2 = rest / E;
+    brick[e] = static_cast<const float*>(a.data)[g0 + d + (uint32_t)m1 * a.PX +
+                                                 (uint32_t)m2 * a.PX * a.PY];
+  }
+  __syncthreads();
+  uint32_t s = 0x9e3779b9u * (blockIdx.x * 256 + threadIdx.x + 1);
+  const float sd = (float)(DP - 1) * (1.0f / 16777216.0f), se = (float)(E - 1) * (1.0f / 16777216.0f);
+  const char* base = reinterpret_cast<const char*>(brick);
+  const uint32_t zbytes = 4u * DP * E;
+  float acc = 0.0f;
+  for (int k = 0; k < nsamp; k += 2) {
+    float q[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      s = s * 1664525u + 1013904223u;
+      q[c] = (float)(s >> 8) * ((c % 3) == 0 ? sd : se);
+    }
+    const float r0 = lds_trilinear<DP>(base, (uint32_t)E, zbytes, q[0], q[1], q[2]);
+    const float r1 = lds_trilinear<DP>(base, (uint32_t)E, zbytes, q[3], q[4], q[5]);
+    acc += r0;
+    acc += r1;
+  }
+  if (acc == 12345.678f) sink[0] = acc;
+}
+
 int max_steps_for(const int vs[3], double h) {
   double diag = sqrt((double)(vs[0] + 2) * (vs[0] + 2) + (double)(vs[1] + 2) * (vs[1] + 2) +
                      (double)(vs[2] + 2) * (vs[2] + 2));
@@ -815,6 +862,8 @@ int launch_adjoint(int device, float* vol, const int vs[3], const double origin[
 // synthetic n^3 volume, best over brick sizes 4, 8 and 16.
 int launch_gather_bench_fp(int device, int dt, int plain, double* samples_per_s) {
   const int n = 256;
+  const bool lds = plain == 2;
+  if (lds) plain = 1;
   const int vs[3] = {n, n, n};
   std::vector<float> host((size_t)n * n * n);
   for (size_t q = 0; q < host.size(); ++q) host[q] = (float)((q * 2654435761u) % 1000) * 1e-3f;
@@ -857,7 +906,10 @@ int launch_gather_bench_fp(int device, int dt, int plain, double* samples_per_s)
   for (int B : {4, 8, 16}) {
     for (int rep = 0; rep < 2; ++rep) {
       FC_HIP(hipEventRecord(e0, nullptr));
-      if (layout == FP_PLAIN) gather_bench_fp<FC_F32, FP_PLAIN><<<blocks, 256>>>(a, B, nsamp, sink);
+      // LDS bricks of 11 x E x E with E = 12, 18, 24 (the projector's typical slab bricks;
+      // 11 * 24^2 floats fit the LDS budget)
+      if (lds) gather_bench_lds<8><<<blocks, 256>>>(a, B == 4 ? 12 : (B == 8 ? 18 : 24), nsamp, sink);
+      else if (layout == FP_PLAIN) gather_bench_fp<FC_F32, FP_PLAIN><<<blocks, 256>>>(a, B, nsamp, sink);
       else if (dt == FC_F32) gather_bench_fp<FC_F32, FP_OCTET><<<blocks, 256>>>(a, B, nsamp, sink);
       else if (dt == FC_BF16) gather_bench_fp<FC_BF16, FP_OCTET><<<blocks, 256>>>(a, B, nsamp, sink);
       else gather_bench_fp<FC_F16, FP_OCTET><<<blocks, 256>>>(a, B, nsamp, sink);

@@ -204,7 +204,8 @@ int fc_set_tile_limits(long long max_voxels, int max_views);
  * interpolation code at pseudo-random positions inside small cache-resident bricks.
  * which = 0: projector trilinear sample, octet layout of `dtype` (8 taps per sample);
  *         1: projector trilinear sample, plain FP32 layout; 2: backprojector bilinear quad gather
- *         of `dtype` (4 taps per sample). Best over several brick sizes. Synchronous. */
+ *         of `dtype` (4 taps per sample); 3: projector trilinear sample from LDS-staged bricks
+ *         (the default projector's fetch). Best over several brick sizes. Synchronous. */
 int fc_gather_bench(int device, int which, int dtype, double* samples_per_s);
 
 /* Kernel timing. When enabled, the main kernel of every fc_* / GPU_CB_* call ("project",
