@@ -49,6 +49,9 @@ def parse():
                                                          "unet_warmstart.npz"),
                     help="trained warm-start network for the CGLS time-to-residual leg")
     ap.add_argument("--no-cgls", action="store_true", help="skip the CGLS leg")
+    ap.add_argument("--copy-calib-mib", type=int, default=0,
+                    help="first run a plain f32 copy (convert_kernel) of this many MiB, 3 times: "
+                         "the known-size reference for HBM byte counters")
     return ap.parse_args()
 
 
@@ -68,6 +71,13 @@ def main():
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.copy_calib_mib > 0:
+        src = torch.rand(args.copy_calib_mib << 18, device=dev)
+        for _ in range(3):
+            ops.convert(src, torch.float32)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {args.copy_calib_mib} MiB read + written per launch, 3 launches")
+        del src
     t_setup = time.time()
     nv = args.views
     if args.random_volume:

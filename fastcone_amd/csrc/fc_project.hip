@@ -234,6 +234,7 @@ struct SlabPlan {
   float rc1[4], rc2[4];
   float margin;
   int nslabs;
+  bool corners_ok;  // all four corner rays advance along the dominant axis
 };
 
 template <int S>
@@ -251,8 +252,9 @@ __device__ __forceinline__ void make_plan(SlabPlan& P, const double* V, const Ra
   const double cdom = ax == 0 ? cdx : (ax == 1 ? cdy : cdz);
   P.ax = ax;
   P.dir = cdom > 0.0 ? 1.0f : -1.0f;
-  P.a1 = ax == 2 ? 0 : ax + 1;
-  P.a2 = ax == 0 ? 2 : (ax == 1 ? 0 : 1);
+  // minor axis 1 is x whenever x is not dominant, so brick columns load coalesced
+  P.a1 = ax == 0 ? 1 : 0;
+  P.a2 = ax == 2 ? 1 : 2;
   auto pick = [](int k, float x, float y, float z) { return k == 0 ? x : (k == 1 ? y : z); };
   P.pd0 = pick(ax, r.px, r.py, r.pz);
   P.p10 = pick(P.a1, r.px, r.py, r.pz);
@@ -273,6 +275,12 @@ __device__ __forceinline__ void make_plan(SlabPlan& P, const double* V, const Ra
   P.src_2 = (float)srcv[P.a2];
   const double jA = (double)j0, jB = (double)min(j0 + kTile - 1, nw - 1);
   const double iA = (double)i0, iB = (double)min(i0 + kTile - 1, nh - 1);
+  // The slice of the tile's frustum between two slab planes is the hull of the corner rays'
+  // intersections only if every ray of the tile crosses the planes forwards. A flat tile's
+  // directions are affine in (j, i), so it suffices that each corner ray advances along the
+  // dominant axis (a corner ray that misses the volume is not checked by its thread's own test);
+  // 0.2 matches that per-ray test.
+  double umin = 1.0;
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     double dx, dy, dz;
@@ -281,7 +289,9 @@ __device__ __forceinline__ void make_plan(SlabPlan& P, const double* V, const Ra
     const double rd = 1.0 / dv[ax];
     P.rc1[c] = (float)(dv[P.a1] * rd);
     P.rc2[c] = (float)(dv[P.a2] * rd);
+    umin = fmin(umin, dv[ax] * (double)P.dir / sqrt(dx * dx + dy * dy + dz * dz));
   }
+  P.corners_ok = umin > 0.2;
   P.margin = V[24] == (double)FC_DET_CYLINDER ? 2.0f : 1.0f;
   P.nslabs = (P.PD - 4 + 1) / S + 1;  // slabs [1 + b S, 1 + (b+1) S) cover [1, n + 2]
 }
@@ -378,7 +388,7 @@ __global__ __launch_bounds__(256) void fp_lds_kernel(FpArgs a) {
   SlabPlan P;
   make_plan<S>(P, V, r, j0, i0, a.nw, a.nh, a.ox, a.oy, a.oz, a.PX, a.PY, a.nz);
   // every active ray must advance along the dominant axis in the same direction
-  const bool bad = r.nk > 0 && !(P.sd * P.dir > 0.2f * (float)a.h);
+  const bool bad = (r.nk > 0 && !(P.sd * P.dir > 0.2f * (float)a.h)) || !P.corners_ok;
   const bool fallback = __syncthreads_or(bad);
 
   float acc = 0.0f;
@@ -403,8 +413,13 @@ __global__ __launch_bounds__(256) void fp_lds_kernel(FpArgs a) {
         continue;
       }
       __syncthreads();  // the previous brick is no longer read
-      // Fill: every load of a chunk is issued before its LDS writes (one round trip per chunk).
-      {
+      // Fill. Dominant axis x (contiguous in HBM and in the brick): element order, every load
+      // of a chunk issued before its LDS writes. Dominant axis y or z: one thread per (minor1,
+      // minor2) column, minor axis 1 being x so a wave's loads stay coalesced, and the DP
+      // elements along the dominant axis unrolled (LDS index d + DP c, pitch DP odd so the
+      // writes are bank-conflict free). Planes past the padded block (last slab) read its last
+      // plane and are never sampled.
+      if (P.gsd == 1u) {
         const float invE1 = 1.0f / (float)B.E1;
         constexpr int kChunk = 8;
 #pragma unroll 1
@@ -422,6 +437,24 @@ __global__ __launch_bounds__(256) void fp_lds_kernel(FpArgs a) {
             const int e = threadIdx.x + 256 * (q0 + q);
             if (e < NE) brick[e] = v[q];
           }
+        }
+      } else {
+        const int ncol = B.E1 * B.E2;
+        const float invE1 = 1.0f / (float)B.E1;
+        const uint32_t g00 = (uint32_t)B.dlo * P.gsd + (uint32_t)B.e1lo * P.gs1 +
+                             (uint32_t)B.e2lo * P.gs2;
+        const int dmax = P.PD - 1 - B.dlo;  // >= 2: dlo = b S <= PD - 3
+#pragma unroll 1
+        for (int c = threadIdx.x; c < ncol; c += 256) {
+          const uint32_t m2 = (uint32_t)(((float)c + 0.5f) * invE1);  // exact: c < 2^16
+          const uint32_t m1 = (uint32_t)c - m2 * (uint32_t)B.E1;
+          const uint32_t g = g00 + m1 * P.gs1 + m2 * P.gs2;
+          float v[DP];
+#pragma unroll
+          for (int d = 0; d < DP; ++d)
+            v[d] = load_elem<DT>(a.data, g + (uint32_t)min(d, dmax) * P.gsd);
+#pragma unroll
+          for (int d = 0; d < DP; ++d) brick[d + DP * c] = v[d];
         }
       }
       __syncthreads();
@@ -514,7 +547,7 @@ __global__ __launch_bounds__(256) void adjoint_lds_kernel(AdjArgs a) {
   if (y == 0.0f) r.nk = 0;
   SlabPlan P;
   make_plan<S>(P, V, r, j0, i0, a.nw, a.nh, a.ox, a.oy, a.oz, a.PX, a.PY, a.nz);
-  const bool bad = r.nk > 0 && !(P.sd * P.dir > 0.2f * (float)a.h);
+  const bool bad = (r.nk > 0 && !(P.sd * P.dir > 0.2f * (float)a.h)) || !P.corners_ok;
   const bool fallback = __syncthreads_or(bad);
   const uint32_t PXu = (uint32_t)a.PX, PXY = (uint32_t)a.PX * (uint32_t)a.PY;
   if (fallback) {

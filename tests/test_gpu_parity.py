@@ -74,11 +74,13 @@ def test_project_layouts_agree(gpu, layout, cap, monkeypatch):
     assert rel(p, ref) < 5e-5
 
 
-def test_project_lds_steep_and_wide_cone(gpu):
+@pytest.mark.parametrize("seed", [14, 33])
+def test_project_lds_steep_and_wide_cone(gpu, seed):
     # short source distance and a tilted detector: large cone angles, views whose dominant axis
-    # is z for some tiles, and bricks that exceed the LDS budget
+    # is z for some tiles, and bricks that exceed the LDS budget; seed 33 has a tile whose corner
+    # ray (missing the volume) points against the tile's dominant axis
     from fastcone_amd import ops
-    rng = np.random.default_rng(14)
+    rng = np.random.default_rng(seed)
     g, prob = _setup(rng, D=28.0, pw=1.6, mis=[0.0, 0.0, 0.0, 0.9, 0.3, 0.2])
     x = rng.standard_normal(g.vol_shape).astype(np.float32)
     p = ops.project(torch.from_numpy(x).to(gpu), g, step=0.5).cpu()
