@@ -129,26 +129,38 @@ def main():
     log(f"f32: step {wall * 1e3:.1f} ms (events {ev_ms:.1f}), FP kernel {fp_ms:.1f} ms, "
         f"BP kernel {bp_ms:.1f} ms")
 
-    # gather roofline of the dominant kernel (the projector: 8 taps per sample)
-    peak_fp = _native.gather_bench(3, _native.FC_F32)   # LDS-brick trilinear, as fp_lds_kernel
+    # gather roofline of the dominant kernel (the projector: 8 taps per sample). The peak is the
+    # projector's own LDS inner loop along ray-coherent paths through a resident brick (no
+    # refill, no divergence); random positions in LDS bricks (bank conflicts) and the LDS
+    # bandwidth (128 B/clk/CU x 256 CUs x 2.4 GHz, one 4-byte tap per lane read) are reported
+    # beside it.
+    peak_fp = _native.gather_bench(4, _native.FC_F32)
+    peak_fp_random = _native.gather_bench(3, _native.FC_F32)
     peak_bp = _native.gather_bench(2, _native.FC_F32)
+    lds_bw_taps = 128.0 * 256 * 2.4e9 / 4.0
     fp_taps = 8.0 * samples
     bp_taps = 4.0 * W
     fp_rate = fp_taps / (fp_ms * 1e-3)
     # compulsory HBM bytes of one projector launch: read the staged (padded) volume once, read
     # and write the projection stack once
     fp_hbm_bytes = (N + 4) ** 3 * 4 + 2 * 4 * nv * NDET * NDET
+    traffic = measured_traffic(nv)
     roofline = {
         "bound": "gather",
         "kernel": "fp_lds_kernel (ray-march projector, LDS slab bricks)",
         "achieved": fp_rate / 1e9, "peak": peak_fp * 8 / 1e9, "unit": "Gtap/s",
         "frac": fp_rate / (peak_fp * 8),
-        "traffic": None,
+        "traffic": None if traffic is None else traffic["bytes_per_launch"],
         "per_unit": "8 taps per ray sample; samples = exact count of the clipped rays",
         "units_per_launch": samples,
+        "peak_kind": "fc_gather_bench(4): the projector's LDS trilinear inner loop on ray-coherent "
+                     "paths through one resident brick",
+        "ceilings": {"lds_random_positions_Gtap_per_s": peak_fp_random * 8 / 1e9,
+                     "lds_bandwidth_Gtap_per_s": lds_bw_taps / 1e9},
         "hbm": {"achieved": fp_hbm_bytes / (fp_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": fp_hbm_bytes / (fp_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                "bytes_per_launch": fp_hbm_bytes},
+                "bytes_per_launch": fp_hbm_bytes,
+                "measured": traffic},
     }
     kernels = {
         "fp": {"ms": fp_ms, "Gvoxel_view_per_s": W / (fp_ms * 1e-3) / 1e9,
@@ -201,6 +213,20 @@ def main():
         "cpu_baseline": cpu, "parity_on_cpu_sample": parity, "cgls_time_to_residual": cgls_leg,
     }
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(nv):
+    """HBM bytes per projector launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    passes (each scaled by what the counter reads for a copy kernel of known size), when they
+    were measured at this view count; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "fp_lds_kernel_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    if int(t.get("views", -1)) != nv:
+        return None
+    return t
 
 
 def cpu_baseline(args, vol, geom, proj, rec, samples):

@@ -758,6 +758,56 @@ __global__ __launch_bounds__(256) void gather_bench_lds(FpArgs a, int E, int nsa
   if (acc == 12345.678f) sink[0] = acc;
 }
 
+// Ray-coherent LDS gather ceiling: the projector's trilinear LDS sample (same lds_trilinear,
+// positions p0 + k s by FMA) over one resident 11 x 16 x 16 brick, with the projector's lane
+// layout (8 x 8 rays per wave, 16 x 16 per block) and ray-like positions (0.75-voxel pixel
+// spacing, diverging directions, 16 samples per slab pass), U samples per iteration. The LDS
+// allocation is exactly the brick (11 KiB), so occupancy is limited by registers only. No refill,
+// no ray set-up and no divergence: it bounds the tap rate of LDS trilinear sampling along rays.
+// Bounds: qd <= 1 + 15 * 0.47 < 9, q1, q2 in [1.25 - 1.125, 1.25 + 11.25 + 1.125 + 0.25] =
+// [0.125, 13.875], so every corner index stays in [0, 9] x [0, 14] x [0, 14].
+template <int S, int U>
+__global__ __launch_bounds__(256) void gather_bench_lds_ray(FpArgs a, int reps,
+                                                            float* __restrict__ sink) {
+  constexpr int DP = S + 3, E = 16, NE = DP * E * E;
+  __shared__ float brick[NE];
+  const int nb = (a.nx - E) / E;
+  const int b = blockIdx.x % (nb * nb);
+  const uint32_t g0 = (uint32_t)(2 + (b % nb) * E) * (uint32_t)a.PX +
+                      (uint32_t)(2 + (b / nb) * E) * (uint32_t)a.PX * (uint32_t)a.PY + 2u;
+  for (int e = threadIdx.x; e < NE; e += 256) {
+    const int d = e % DP, rest = e / DP, m1 = rest % E, m2 = rest / E;
+    brick[e] = static_cast<const float*>(a.data)[g0 + d + (uint32_t)m1 * a.PX +
+                                                 (uint32_t)m2 * a.PX * a.PY];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const float jj = (float)((wave & 1) * 8 + (lane & 7)), ii = (float)((wave >> 1) * 8 + (lane >> 3));
+  const float sd = 0.47f, s1 = 0.01f * (jj - 7.5f), s2 = 0.01f * (ii - 7.5f);
+  const float pd0 = 1.0f;
+  const char* base = reinterpret_cast<const char*>(brick);
+  const uint32_t zbytes = 4u * DP * E;
+  float acc = 0.0f;
+#pragma unroll 1
+  for (int rep = 0; rep < reps; ++rep) {
+    const float off = (float)(rep & 1) * 0.25f;  // keeps passes distinct for the compiler
+    const float p10 = 1.25f + 0.75f * jj + off, p20 = 1.25f + 0.75f * ii;
+#pragma unroll 1
+    for (int k = 0; k < 16; k += U) {
+      float r[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const float kf = (float)(k + u);
+        r[u] = lds_trilinear<DP>(base, (uint32_t)E, zbytes, fmaf(kf, sd, pd0),
+                                 fmaf(kf, s1, p10), fmaf(kf, s2, p20));
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) acc += r[u];
+    }
+  }
+  if (acc == 12345.678f) sink[0] = acc;
+}
+
 int max_steps_for(const int vs[3], double h) {
   double diag = sqrt((double)(vs[0] + 2) * (vs[0] + 2) + (double)(vs[1] + 2) * (vs[1] + 2) +
                      (double)(vs[2] + 2) * (vs[2] + 2));
@@ -895,7 +945,7 @@ int launch_adjoint(int device, float* vol, const int vs[3], const double origin[
 // synthetic n^3 volume, best over brick sizes 4, 8 and 16.
 int launch_gather_bench_fp(int device, int dt, int plain, double* samples_per_s) {
   const int n = 256;
-  const bool lds = plain == 2;
+  const bool lds = plain >= 2, ray = plain == 3;
   if (lds) plain = 1;
   const int vs[3] = {n, n, n};
   std::vector<float> host((size_t)n * n * n);
@@ -941,7 +991,9 @@ int launch_gather_bench_fp(int device, int dt, int plain, double* samples_per_s)
       FC_HIP(hipEventRecord(e0, nullptr));
       // LDS bricks of 11 x E x E with E = 12, 18, 24 (the projector's typical slab bricks;
       // 11 * 24^2 floats fit the LDS budget)
-      if (lds) gather_bench_lds<8><<<blocks, 256>>>(a, B == 4 ? 12 : (B == 8 ? 18 : 24), nsamp, sink);
+      if (ray && B == 4) gather_bench_lds_ray<8, 2><<<blocks, 256>>>(a, nsamp / 16, sink);
+      else if (ray) gather_bench_lds_ray<8, 4><<<blocks, 256>>>(a, nsamp / 16, sink);
+      else if (lds) gather_bench_lds<8><<<blocks, 256>>>(a, B == 4 ? 12 : (B == 8 ? 18 : 24), nsamp, sink);
       else if (layout == FP_PLAIN) gather_bench_fp<FC_F32, FP_PLAIN><<<blocks, 256>>>(a, B, nsamp, sink);
       else if (dt == FC_F32) gather_bench_fp<FC_F32, FP_OCTET><<<blocks, 256>>>(a, B, nsamp, sink);
       else if (dt == FC_BF16) gather_bench_fp<FC_BF16, FP_OCTET><<<blocks, 256>>>(a, B, nsamp, sink);
@@ -950,7 +1002,8 @@ int launch_gather_bench_fp(int device, int dt, int plain, double* samples_per_s)
       FC_HIP(hipEventSynchronize(e1));
       float ms = 0.f;
       FC_HIP(hipEventElapsedTime(&ms, e0, e1));
-      double rate = (double)blocks * 256.0 * nsamp / (ms * 1e-3);
+      const double per_thread = ray ? (double)(nsamp / 16) * 16.0 : (double)nsamp;
+      double rate = (double)blocks * 256.0 * per_thread / (ms * 1e-3);
       if (rep == 1 && rate > best) best = rate;
     }
   }
