@@ -136,7 +136,8 @@ def main():
     # beside it.
     peak_fp = _native.gather_bench(4, _native.FC_F32)
     peak_fp_random = _native.gather_bench(3, _native.FC_F32)
-    peak_bp = _native.gather_bench(2, _native.FC_F32)
+    peak_bp = _native.gather_bench(5, _native.FC_F32)         # ray-coherent BP inner loop
+    peak_bp_random = _native.gather_bench(2, _native.FC_F32)
     lds_bw_taps = 128.0 * 256 * 2.4e9 / 4.0
     fp_taps = 8.0 * samples
     bp_taps = 4.0 * W
@@ -168,7 +169,8 @@ def main():
         "bp": {"ms": bp_ms, "Gvoxel_view_per_s": W / (bp_ms * 1e-3) / 1e9,
                "Gtap_per_s": bp_taps / (bp_ms * 1e-3) / 1e9,
                "gather_peak_Gtap_per_s": peak_bp * 4 / 1e9,
-               "gather_frac": bp_taps / (bp_ms * 1e-3) / (peak_bp * 4)},
+               "gather_frac": bp_taps / (bp_ms * 1e-3) / (peak_bp * 4),
+               "gather_random_positions_Gtap_per_s": peak_bp_random * 4 / 1e9},
         "R_kernels_only": W / ((fp_ms + bp_ms) * 1e-3) / 1e9,
     }
 

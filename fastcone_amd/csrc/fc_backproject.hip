@@ -262,7 +262,49 @@ int launch_stage_quads(const void* proj, int dt, int nw, int nh, int nv, const f
   return FC_OK;
 }
 
-int launch_gather_bench_bp(int device, int dt, double* samples_per_s) {
+// Ray-coherent ceiling for the backprojector: its fast-path inner loop (per column: one u and
+// weight per view, 16 z voxels whose rows advance by dv; clamp, bilinear quad gather, weighted
+// accumulate) over one 64 x 64 pixel patch per block that stays cache resident, with the
+// backprojector's lane layout and view-to-view drift. No per-view projective set-up and no fresh
+// projection data per view, so it bounds the voxel-view rate of the gather itself.
+// Bounds: u <= bu + 15.35, v <= bv + 4.85 + 15 * 1.3 = bv + 24.35, inside the 64 x 64 patch.
+template <int DT>
+__global__ __launch_bounds__(256) void gather_bench_bp_ray(const void* quad, int QW, int QH,
+                                                           int nviews, float* __restrict__ sink) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const float xl = (float)((wave & 1) * 8 + (lane & 7)), yl = (float)((wave >> 1) * 8 + (lane >> 3));
+  const int npx = (QW - 64) / 64, npy = (QH - 64) / 64;
+  const int b = blockIdx.x % (npx * npy);
+  const float bu = (float)((b % npx) * 64 + 2), bv = (float)((b / npx) * 64 + 2);
+  const float hmax = (float)(QH - 2);
+  float acc[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) acc[k] = 0.0f;
+#pragma unroll 1
+  for (int t = 0; t < nviews; ++t) {
+    const float tt = (float)(t & 7);
+    const float u = bu + 0.7f * xl + 0.3f * yl + 0.05f * tt;
+    const float v0 = bv + 0.2f * xl + 0.1f * yl + 0.05f * tt;
+    const float w = 1.0f + 0.001f * tt;
+    const float fu = floorf(u);
+    const float au = u - fu;
+    const uint32_t iu = (uint32_t)(int)fu;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      float v = clampf(fmaf((float)k, 1.3f, v0), -1.0f, hmax);
+      float fv = floorf(v);
+      float av = v - fv;
+      uint32_t idx = (uint32_t)(int)fv * (uint32_t)QW + iu;
+      acc[k] = fmaf(w, bilinear_quad<DT>(quad, idx, au, av), acc[k]);
+    }
+  }
+  float sum = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) sum += acc[k];
+  if (sum == 12345.678f) sink[0] = sum;
+}
+
+int launch_gather_bench_bp(int device, int dt, int ray, double* samples_per_s) {
   (void)device;
   const int QW = 1024, QH = 1024;
   size_t esz = (dt == FC_F32) ? 16 : 8;
@@ -279,7 +321,10 @@ int launch_gather_bench_bp(int device, int dt, double* samples_per_s) {
   for (int B : {8, 16, 32}) {
     for (int rep = 0; rep < 2; ++rep) {
       FC_HIP(hipEventRecord(e0, nullptr));
-      if (dt == FC_F32) gather_bench_bp<FC_F32><<<blocks, 256>>>(quad, QW, QH, B, nsamp, sink);
+      if (ray && dt == FC_F32) gather_bench_bp_ray<FC_F32><<<blocks, 256>>>(quad, QW, QH, nsamp / 16, sink);
+      else if (ray && dt == FC_BF16) gather_bench_bp_ray<FC_BF16><<<blocks, 256>>>(quad, QW, QH, nsamp / 16, sink);
+      else if (ray) gather_bench_bp_ray<FC_F16><<<blocks, 256>>>(quad, QW, QH, nsamp / 16, sink);
+      else if (dt == FC_F32) gather_bench_bp<FC_F32><<<blocks, 256>>>(quad, QW, QH, B, nsamp, sink);
       else if (dt == FC_BF16) gather_bench_bp<FC_BF16><<<blocks, 256>>>(quad, QW, QH, B, nsamp, sink);
       else gather_bench_bp<FC_F16><<<blocks, 256>>>(quad, QW, QH, B, nsamp, sink);
       FC_HIP(hipEventRecord(e1, nullptr));

@@ -88,7 +88,7 @@ int launch_sample_count(int device, const int vol_shape[3], const double origin[
 int launch_stage_quads(const void* proj, int dt, int nw, int nh, int nv, const float* norm,
                        void* quad, hipStream_t stream);
 int launch_gather_bench_fp(int device, int dt, int plain, double* samples_per_s);
-int launch_gather_bench_bp(int device, int dt, double* samples_per_s);
+int launch_gather_bench_bp(int device, int dt, int ray, double* samples_per_s);
 int launch_convert(const void* src, int src_dtype, void* dst, int dst_dtype, long long n,
                    hipStream_t stream);
 

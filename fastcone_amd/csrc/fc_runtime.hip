@@ -221,13 +221,14 @@ int fc_device_count(void) {
 int fc_gather_bench(int device, int which, int dtype, double* samples_per_s) {
   clear_error();
   FC_CHECK_ARG(samples_per_s, "null output");
-  FC_CHECK_ARG(which >= 0 && which <= 4, "unknown gather benchmark %d", which);
+  FC_CHECK_ARG(which >= 0 && which <= 5, "unknown gather benchmark %d", which);
   FC_CHECK_ARG(dtype_ok(dtype), "unknown dtype %d", dtype);
-  FC_CHECK_ARG(which == 0 || which == 2 || dtype == FC_F32,
+  FC_CHECK_ARG(which == 0 || which == 2 || which == 5 || dtype == FC_F32,
                "the plain and LDS benchmarks are FP32 only");
   int rc = select_device(device);
   if (rc) return rc;
-  if (which == 2) return launch_gather_bench_bp(device, dtype, samples_per_s);
+  if (which == 2 || which == 5)
+    return launch_gather_bench_bp(device, dtype, which == 5, samples_per_s);
   return launch_gather_bench_fp(device, dtype, which == 0 ? 0 : (which == 1 ? 1 : which - 1),
                                 samples_per_s);
 }

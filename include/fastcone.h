@@ -207,7 +207,9 @@ int fc_set_tile_limits(long long max_voxels, int max_views);
  *            `dtype` (4 taps per sample), random positions; 3: projector trilinear sample from
  *            LDS bricks at random positions (FP32); 4: the LDS projector's inner loop along
  *            ray-coherent paths through one resident brick (FP32; no refill, no divergence: the
- *            ceiling of the default projector). Best over several runs. Synchronous. */
+ *            ceiling of the default projector); 5: the backprojector's inner loop (per column one
+ *            u per view, 16 z rows) over one cache-resident patch of `dtype` quads (4 taps per
+ *            sample). Best over several runs. Synchronous. */
 int fc_gather_bench(int device, int which, int dtype, double* samples_per_s);
 
 /* Kernel timing. When enabled, the main kernel of every fc_* / GPU_CB_* call ("project",
