@@ -39,8 +39,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-variants", action="store_true", help="skip BF16/FP16 storage runs")
-    ap.add_argument("--cpu-views", type=int, default=2)
-    ap.add_argument("--cpu-slices", type=int, default=4)
+    ap.add_argument("--cpu-views", type=int, default=24)
+    ap.add_argument("--cpu-slices", type=int, default=16)
     ap.add_argument("--views", type=int, default=NV,
                     help="views of the workload (default 720; fewer only for profiling runs)")
     ap.add_argument("--random-volume", action="store_true",

@@ -10,19 +10,30 @@
 // interpolant of window t extended by zeros; the per-view norm multiplies the data at staging,
 // as the reference's fillCache does (coneBeamGPU.cu:1081).
 //
-// Work decomposition: one thread owns a column (x, y) and ZR consecutive z voxels, held in
-// registers over every view ("wavefront-level accumulation over views"): per view and column the
-// projective terms are evaluated once in FP64; when the detector columns do not depend on z (all
-// untilted detectors) each voxel then costs one FMA for its row, a clamp and one bilinear gather.
+// Work decomposition: one thread owns a column (x, y) and ZR = 16 consecutive z voxels, held in
+// registers over every view ("wavefront-level accumulation over views"); a block owns a
+// 16 x 16 x 16 voxel cube. Per view the block stages, in LDS, the detector window that the cube
+// projects onto (rows of the raw projection, zero outside the detector), double-buffered: while
+// the block accumulates view t from one buffer, the rows of view t+1 are already in flight into
+// registers and are written to the other buffer at the top of the next iteration, so one barrier
+// per view suffices and the load latency hides behind the arithmetic. Per view and column the
+// projective terms are evaluated once in FP64; when the detector columns do not depend on z
+// (untilted detectors) each voxel then costs one FMA for its row, a clamp and a bilinear
+// interpolation from LDS (two ds_read2 pairs).
 //
-// Projection staging ("quad" layout): for every base pixel (r, c) in [-1, nh] x [-1, nw] the four
-// bilinear corners p[r..r+1][c..c+1] * norm_t are stored together (16 B FP32, 8 B BF16/FP16), so a
-// gather is one load; rows and columns are clamped to the padded range, which keeps every load in
-// bounds and returns exact zeros outside the window.
+// The windows come from a pre-pass (bp_windows, one thread per block and view): the projective
+// image of a box lies in the box of its eight corners' images, so the corners' FP64 (u, v) widened
+// by a pixel each side bound every tap. A view whose window does not fit the LDS tile is gathered
+// from global memory with explicit bounds (rare: very short source distances); a view that misses
+// the detector entirely is skipped.
+//
+// "Quad" staging (stage_quads: the four bilinear corners of every base pixel stored together) is
+// kept for the Katsevich backprojector's filtered data.
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 
 #include "fc_device.h"
 #include "fc_internal.h"
@@ -32,21 +43,6 @@ namespace {
 
 constexpr int kZR = 16;   // z voxels per thread
 constexpr int kBXY = 16;  // 16x16 columns per 256-thread block (8x8 per wave)
-
-template <int DT>
-struct Quad;
-template <>
-struct Quad<FC_F32> {
-  using T = float4;
-};
-template <>
-struct Quad<FC_BF16> {
-  using T = uint2;
-};
-template <>
-struct Quad<FC_F16> {
-  using T = uint2;
-};
 
 template <int DT_IN>
 __global__ __launch_bounds__(256) void stage_quads(const void* __restrict__ proj, int nw, int nh,
@@ -73,29 +69,94 @@ __global__ __launch_bounds__(256) void stage_quads(const void* __restrict__ proj
   }
 }
 
+constexpr int kTileW = 64;                 // widest staged window (one pixel per lane)
+constexpr int kPre = 12;                   // rows per wave and view: window height <= 4 kPre
+constexpr int kTileCap = kTileW * 4 * kPre;  // floats per LDS buffer (12 KiB)
+constexpr uint32_t kWinSkip = 0u;          // window word y: the cube misses the detector
+constexpr uint32_t kWinGlobal = 0xffffu;   // window word y low half: gather from global memory
+
 struct BpArgs {
   float* vol;
-  const void* quad;
+  const void* proj;    // [nviews][nh][nw] in the storage dtype
+  const float* norm;   // per-view scale or null
   const double* views;
+  const uint2* win;    // per (block, view): x = u0 | v0 << 16 (int16 each), y = Wu | Wv << 16
   double ox, oy, oz;
   int nx, ny, nz;
   int nw, nh, nviews;
-  int QW, QH;
+  int nbx, nby;
 };
 
-template <int DT>
-__device__ __forceinline__ float bilinear_quad(const void* base, uint32_t idx, float au,
-                                               float av) {
-  float p00, p01, p10, p11;
-  if constexpr (DT == FC_F32) {
-    float4 q = static_cast<const float4*>(base)[idx];
-    p00 = q.x; p01 = q.y; p10 = q.z; p11 = q.w;
-  } else {
-    uint2 q = static_cast<const uint2*>(base)[idx];
-    p00 = lo16<DT>(q.x); p01 = hi16<DT>(q.x); p10 = lo16<DT>(q.y); p11 = hi16<DT>(q.y);
+struct WinArgs {
+  const double* views;
+  uint2* win;
+  double ox, oy, oz;
+  int nx, ny, nz;
+  int nw, nh, nviews;
+  int nbx, nby, nbz;
+};
+
+// One thread per (block, view): the detector window of the block's voxel cube. Window pixels
+// [u0, u0 + Wu) x [v0, v0 + Wv) with u0 = floor(min u) - 1 and Wu = floor(max u) - u0 + 3, so every
+// tap (iu, iu + 1) of a voxel in the cube lies at least one pixel inside; likewise v.
+__global__ __launch_bounds__(256) void bp_windows(WinArgs a) {
+  const size_t total = (size_t)a.nbx * a.nby * a.nbz * a.nviews;
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= total) return;
+  const int t = (int)(idx % (size_t)a.nviews);
+  const size_t b = idx / (size_t)a.nviews;
+  const int bx = (int)(b % a.nbx), by = (int)((b / a.nbx) % a.nby), bz = (int)(b / ((size_t)a.nbx * a.nby));
+  const double xs[2] = {bx * kBXY + a.ox, min(bx * kBXY + kBXY - 1, a.nx - 1) + a.ox};
+  const double ys[2] = {by * kBXY + a.oy, min(by * kBXY + kBXY - 1, a.ny - 1) + a.oy};
+  const double zs[2] = {bz * kZR + a.oz, min(bz * kZR + kZR - 1, a.nz - 1) + a.oz};
+  const double* M = a.views + (size_t)t * FC_VIEW_DOUBLES + 12;
+  double umin = 1e300, umax = -1e300, vmin = 1e300, vmax = -1e300;
+  bool ok = true;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const double X = xs[c & 1], Y = ys[(c >> 1) & 1], Z = zs[c >> 2];
+    const double C = M[8] * X + M[9] * Y + M[10] * Z + M[11];
+    ok = ok && C > 0.0;
+    const double u = (M[0] * X + M[1] * Y + M[2] * Z + M[3]) / C;
+    const double v = (M[4] * X + M[5] * Y + M[6] * Z + M[7]) / C;
+    umin = fmin(umin, u); umax = fmax(umax, u);
+    vmin = fmin(vmin, v); vmax = fmax(vmax, v);
   }
-  float top = lerp(p00, p01, au), bot = lerp(p10, p11, au);
-  return lerp(top, bot, av);
+  uint2 w;
+  w.x = 0u;
+  w.y = kWinGlobal;
+  if (ok && umin > -1e6 && umax < 1e6 && vmin > -1e6 && vmax < 1e6) {
+    if (umax <= -1.0 || umin >= (double)a.nw || vmax <= -1.0 || vmin >= (double)a.nh) {
+      w.y = kWinSkip;  // zero-extended data: every sample of the cube is 0
+    } else {
+      const int u0 = (int)floor(umin) - 1, v0 = (int)floor(vmin) - 1;
+      const int Wu = (int)floor(umax) - u0 + 3, Wv = (int)floor(vmax) - v0 + 3;
+      if (Wu <= kTileW && Wv <= 4 * kPre && u0 >= -32768 && u0 <= 32767 && v0 >= -32768 &&
+          v0 <= 32767) {
+        w.x = ((uint32_t)u0 & 0xffffu) | ((uint32_t)v0 << 16);
+        w.y = (uint32_t)Wu | ((uint32_t)Wv << 16);
+      }
+    }
+  }
+  a.win[idx] = w;
+}
+
+// Zero-extended bilinear sample of one view straight from global memory (window fallback).
+template <int DT>
+__device__ __forceinline__ float bilinear_global(const void* proj, size_t vbase, int nw, int nh,
+                                                 float u, float v) {
+  if (!(u > -1.0f && u < (float)nw && v > -1.0f && v < (float)nh)) return 0.0f;
+  const float fu = floorf(u), fv = floorf(v);
+  const int iu = (int)fu, iv = (int)fv;
+  const float au = u - fu, av = v - fv;
+  float p[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int cu = iu + (q & 1), cv = iv + (q >> 1);
+    const bool in = cu >= 0 && cu < nw && cv >= 0 && cv < nh;
+    p[q] = in ? load_elem<DT>(proj, vbase + (size_t)cv * nw + cu) : 0.0f;
+  }
+  return lerp(lerp(p[0], p[1], au), lerp(p[2], p[3], au), av);
 }
 
 // Clamp to [lo, hi]; NaN maps to lo (fmaxf returns the non-NaN operand).
@@ -103,67 +164,130 @@ __device__ __forceinline__ float clampf(float v, float lo, float hi) {
   return fminf(fmaxf(v, lo), hi);
 }
 
+// Bilinear sample at tile coordinates (u, v) >= 0 of an LDS window with row pitch Wu.
+__device__ __forceinline__ float tile_bilinear(const float* T, uint32_t Wu, uint32_t iu, float au,
+                                               float v) {
+  const uint32_t iv = (uint32_t)(int)v;
+  const float av = __builtin_amdgcn_fractf(v);
+  const float* p = T + mad_u24(iv, Wu, iu);
+  const float* q = p + Wu;
+  return lerp(lerp(p[0], p[1], au), lerp(q[0], q[1], au), av);
+}
+
 template <int DT, int MODE>
 __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
+  __shared__ float tile[2][kTileCap];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int x = blockIdx.x * kBXY + (wave & 1) * 8 + (lane & 7);
   const int y = blockIdx.y * kBXY + (wave >> 1) * 8 + (lane >> 3);
   const int z0 = blockIdx.z * kZR;
   const double X = (double)x + a.ox, Y = (double)y + a.oy, Z0 = (double)z0 + a.oz;
-  const float wmax = (float)a.nw, hmax = (float)a.nh;
-  const size_t vstride = (size_t)a.QW * a.QH;
+  const uint2* W = a.win + ((size_t)(blockIdx.z * a.nby + blockIdx.y) * a.nbx + blockIdx.x) *
+                               (size_t)a.nviews;
   float acc[kZR];
 #pragma unroll
   for (int k = 0; k < kZR; ++k) acc[k] = 0.0f;
 
+  // Rows of a view's window, staged as (row wave + 4 q, column lane); zero outside the detector.
+  float pre[kPre];
+  auto issue = [&](int t, uint2 w) {
+    const uint32_t Wu = w.y & 0xffffu, Wv = w.y >> 16;
+    if (Wu == kWinSkip || Wu == kWinGlobal) return;
+    const int u0 = (int)(int16_t)(w.x & 0xffffu), v0 = (int)(int16_t)(w.x >> 16);
+    const int pu = u0 + lane;
+    const bool colok = (uint32_t)lane < Wu && pu >= 0 && pu < a.nw;
+    const size_t vbase = (size_t)t * a.nh * a.nw;
+#pragma unroll
+    for (int q = 0; q < kPre; ++q) {
+      const int r = wave + 4 * q;
+      const int pv = v0 + r;
+      pre[q] = 0.0f;
+      if (colok && (uint32_t)r < Wv && pv >= 0 && pv < a.nh)
+        pre[q] = load_elem<DT>(a.proj, vbase + (size_t)pv * a.nw + pu);
+    }
+  };
+  auto commit = [&](float* T, uint2 w) {
+    const uint32_t Wu = w.y & 0xffffu, Wv = w.y >> 16;
+    if (Wu == kWinSkip || Wu == kWinGlobal) return;
+#pragma unroll
+    for (int q = 0; q < kPre; ++q) {
+      const uint32_t r = (uint32_t)(wave + 4 * q);
+      if ((uint32_t)lane < Wu && r < Wv) T[r * Wu + (uint32_t)lane] = pre[q];
+    }
+  };
+
+  uint2 wn = make_uint2(0u, kWinSkip);
+  if (a.nviews > 0) {
+    wn = W[0];
+    issue(0, wn);
+  }
   for (int t = 0; t < a.nviews; ++t) {
+    const uint2 w = wn;
+    float* T = tile[t & 1];
+    // the other buffer's readers (view t-1) all passed the previous barrier before this write
+    commit(T, w);
+    __syncthreads();
+    if (t + 1 < a.nviews) {
+      wn = W[t + 1];
+      issue(t + 1, wn);  // in flight while view t is accumulated
+    }
+    const uint32_t Wu = w.y & 0xffffu, Wv = w.y >> 16;
+    if (Wu == kWinSkip) continue;  // uniform: the cube misses the detector in this view
     const double* M = a.views + (size_t)t * FC_VIEW_DOUBLES + 12;
     const double A = M[0] * X + M[1] * Y + M[2] * Z0 + M[3];
     const double B = M[4] * X + M[5] * Y + M[6] * Z0 + M[7];
     const double C = M[8] * X + M[9] * Y + M[10] * Z0 + M[11];
-    const void* base;
-    if constexpr (DT == FC_F32) {
-      base = static_cast<const float4*>(a.quad) + (size_t)t * vstride;
-    } else {
-      base = static_cast<const uint2*>(a.quad) + (size_t)t * vstride;
+    const float s = a.norm ? a.norm[t] : 1.0f;
+    if (Wu == kWinGlobal) {
+      const size_t vbase = (size_t)t * a.nh * a.nw;
+      const float af = (float)A, bf = (float)B, cf = (float)C;
+      const float da = (float)M[2], db = (float)M[6], dc = (float)M[10];
+#pragma unroll
+      for (int k = 0; k < kZR; ++k) {
+        const float kf = (float)k;
+        const float rc = 1.0f / fmaf(kf, dc, cf);
+        float wt = s;
+        if constexpr (MODE == FC_BP_FDK) wt = s * rc * rc;
+        if constexpr (MODE == FC_BP_KATSEVICH) wt = s * rc;
+        acc[k] = fmaf(wt, bilinear_global<DT>(a.proj, vbase, a.nw, a.nh, fmaf(kf, da, af) * rc,
+                                              fmaf(kf, db, bf) * rc), acc[k]);
+      }
+      continue;
     }
+    const float u0f = (float)(int16_t)(w.x & 0xffffu), v0f = (float)(int16_t)(w.x >> 16);
+    // Tile coordinates. Valid voxels lie >= 1 pixel inside the window; the clamps only keep
+    // voxels outside the volume (partial blocks) and NaN inside the buffer.
+    const float umax = (float)Wu - 2.0f, vmax = (float)Wv - 2.0f;
     if (M[2] == 0.0 && M[10] == 0.0) {  // uniform per view
       // Columns independent of z: u and the weight are per column, the row is linear in z.
       const double rc = 1.0 / C;
-      float u = clampf((float)(A * rc), -1.0f, wmax);
-      const float v0 = (float)(B * rc);
+      const float u = clampf((float)(A * rc) - u0f, 0.0f, umax);
+      const float vb = (float)(B * rc) - v0f;
       const float dv = (float)(M[6] * rc);
-      float w = 1.0f;
-      if constexpr (MODE == FC_BP_FDK) w = (float)(rc * rc);
-      if constexpr (MODE == FC_BP_KATSEVICH) w = (float)rc;
-      const float fu = floorf(u);
-      const float au = u - fu;
-      const uint32_t iu = (uint32_t)((int)fu + 1);
+      float wt = s;
+      if constexpr (MODE == FC_BP_FDK) wt = s * (float)(rc * rc);
+      if constexpr (MODE == FC_BP_KATSEVICH) wt = s * (float)rc;
+      const uint32_t iu = (uint32_t)(int)u;
+      const float au = __builtin_amdgcn_fractf(u);
 #pragma unroll
       for (int k = 0; k < kZR; ++k) {
-        float v = clampf(fmaf((float)k, dv, v0), -1.0f, hmax);
-        float fv = floorf(v);
-        float av = v - fv;
-        uint32_t idx = (uint32_t)((int)fv + 1) * (uint32_t)a.QW + iu;
-        acc[k] = fmaf(w, bilinear_quad<DT>(base, idx, au, av), acc[k]);
+        const float v = clampf(fmaf((float)k, dv, vb), 0.0f, vmax);
+        acc[k] = fmaf(wt, tile_bilinear(T, Wu, iu, au, v), acc[k]);
       }
     } else {
       const float af = (float)A, bf = (float)B, cf = (float)C;
       const float da = (float)M[2], db = (float)M[6], dc = (float)M[10];
 #pragma unroll
       for (int k = 0; k < kZR; ++k) {
-        float kf = (float)k;
-        float c = fmaf(kf, dc, cf);
-        float rc = 1.0f / c;
-        float u = clampf(fmaf(kf, da, af) * rc, -1.0f, wmax);
-        float v = clampf(fmaf(kf, db, bf) * rc, -1.0f, hmax);
-        float fu = floorf(u), fv = floorf(v);
-        float au = u - fu, av = v - fv;
-        uint32_t idx = (uint32_t)((int)fv + 1) * (uint32_t)a.QW + (uint32_t)((int)fu + 1);
-        float w = 1.0f;
-        if constexpr (MODE == FC_BP_FDK) w = rc * rc;
-        if constexpr (MODE == FC_BP_KATSEVICH) w = rc;
-        acc[k] = fmaf(w, bilinear_quad<DT>(base, idx, au, av), acc[k]);
+        const float kf = (float)k;
+        const float rc = 1.0f / fmaf(kf, dc, cf);
+        const float u = clampf(fmaf(kf, da, af) * rc - u0f, 0.0f, umax);
+        const float v = clampf(fmaf(kf, db, bf) * rc - v0f, 0.0f, vmax);
+        float wt = s;
+        if constexpr (MODE == FC_BP_FDK) wt = s * rc * rc;
+        if constexpr (MODE == FC_BP_KATSEVICH) wt = s * rc;
+        acc[k] = fmaf(wt, tile_bilinear(T, Wu, (uint32_t)(int)u, __builtin_amdgcn_fractf(u), v),
+                      acc[k]);
       }
     }
   }
@@ -176,27 +300,64 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
   }
 }
 
-// Gather-roofline microbenchmark for the backprojector: bilinear quad gathers at pseudo-random
-// positions inside one B x B pixel patch per block (same fetch + interpolation code).
-template <int DT>
-__global__ __launch_bounds__(256) void gather_bench_bp(const void* quad, int QW, int QH, int B,
+// Gather-roofline microbenchmark for the backprojector: bilinear samples from one staged 64 x 48
+// LDS window per block at pseudo-random positions (same tile_bilinear as bp_kernel).
+__global__ __launch_bounds__(256) void gather_bench_bp(const float* __restrict__ src, int pitch,
                                                        int nsamp, float* __restrict__ sink) {
+  __shared__ float T[kTileCap];
+  const float* g = src + (size_t)(blockIdx.x % 64) * 48 * pitch;
+  for (int e = threadIdx.x; e < kTileCap; e += 256) T[e] = g[(e / kTileW) * pitch + e % kTileW];
+  __syncthreads();
   uint32_t s = 0x9e3779b9u * (blockIdx.x * 256 + threadIdx.x + 1);
-  int npx = (QW - B - 1) / B, npy = (QH - B - 1) / B;
-  int b = blockIdx.x % (npx * npy);
-  float bu = (float)((b % npx) * B), bv = (float)((b / npx) * B);
-  const float scale = (float)(B - 1) * (1.0f / 16777216.0f);
+  const float su = 62.0f / 16777216.0f, sv = 46.0f / 16777216.0f;
   float acc = 0.0f;
   for (int k = 0; k < nsamp; ++k) {
     s = s * 1664525u + 1013904223u;
-    float u = bu + (float)(s >> 8) * scale;
+    const float u = (float)(s >> 8) * su;
     s = s * 1664525u + 1013904223u;
-    float v = bv + (float)(s >> 8) * scale;
-    float fu = floorf(u), fv = floorf(v);
-    uint32_t idx = (uint32_t)(int)fv * (uint32_t)QW + (uint32_t)(int)fu;
-    acc = fmaf(1.0001f, bilinear_quad<DT>(quad, idx, u - fu, v - fv), acc);
+    const float v = (float)(s >> 8) * sv;
+    acc = fmaf(1.0001f, tile_bilinear(T, kTileW, (uint32_t)(int)u, __builtin_amdgcn_fractf(u), v),
+               acc);
   }
   if (acc == 12345.678f) sink[0] = acc;
+}
+
+// Ray-coherent ceiling for the backprojector: bp_kernel's fast-path inner loop (per column one u
+// and weight per view, 16 z voxels whose rows advance by dv: clamp, bilinear sample from the LDS
+// window, weighted accumulate) over one resident 64 x 48 window per block, with the
+// backprojector's lane layout and view-to-view drift. No per-view projective set-up, no window
+// staging and no barriers, so it bounds the voxel-view rate of the LDS gather itself.
+// Bounds: u <= 2 + 10.5 + 4.5 + 0.35 < 62, v <= 2 + 3 + 1.5 + 0.35 + 15 * 1.3 < 46.
+__global__ __launch_bounds__(256) void gather_bench_bp_ray(const float* __restrict__ src,
+                                                           int pitch, int nviews,
+                                                           float* __restrict__ sink) {
+  __shared__ float T[kTileCap];
+  const float* g = src + (size_t)(blockIdx.x % 64) * 48 * pitch;
+  for (int e = threadIdx.x; e < kTileCap; e += 256) T[e] = g[(e / kTileW) * pitch + e % kTileW];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const float xl = (float)((wave & 1) * 8 + (lane & 7)), yl = (float)((wave >> 1) * 8 + (lane >> 3));
+  float acc[kZR];
+#pragma unroll
+  for (int k = 0; k < kZR; ++k) acc[k] = 0.0f;
+#pragma unroll 1
+  for (int t = 0; t < nviews; ++t) {
+    const float tt = (float)(t & 7);
+    const float u = clampf(2.0f + 0.7f * xl + 0.3f * yl + 0.05f * tt, 0.0f, 62.0f);
+    const float vb = 2.0f + 0.2f * xl + 0.1f * yl + 0.05f * tt;
+    const float wt = 1.0f + 0.001f * tt;
+    const uint32_t iu = (uint32_t)(int)u;
+    const float au = __builtin_amdgcn_fractf(u);
+#pragma unroll
+    for (int k = 0; k < kZR; ++k) {
+      const float v = clampf(fmaf((float)k, 1.3f, vb), 0.0f, 46.0f);
+      acc[k] = fmaf(wt, tile_bilinear(T, kTileW, iu, au, v), acc[k]);
+    }
+  }
+  float sum = 0.0f;
+#pragma unroll
+  for (int k = 0; k < kZR; ++k) sum += acc[k];
+  if (sum == 12345.678f) sink[0] = sum;
 }
 
 template <int DT>
@@ -214,38 +375,40 @@ int launch_backproject(int device, float* vol, const int vs[3], const double ori
                        const void* proj, int dt, const int ps[3], const double* views,
                        const float* norm, int mode, hipStream_t stream) {
   const int nw = ps[0], nh = ps[1], nv = ps[2];
-  const int QW = nw + 2, QH = nh + 2;
-  FC_CHECK_ARG(QH <= 65535 && nv <= 65535, "projection stack too large for the staging grid");
-  FC_CHECK_ARG((size_t)QW * QH < (1ull << 31), "projection too large for 32-bit offsets");
-  size_t esz = (dt == FC_F32) ? 16 : 8;
-  size_t bytes = (size_t)QW * QH * nv * esz;
+  if (nv == 0 || vs[0] == 0 || vs[1] == 0 || vs[2] == 0) return FC_OK;
+  const int nbx = (vs[0] + kBXY - 1) / kBXY, nby = (vs[1] + kBXY - 1) / kBXY,
+            nbz = (vs[2] + kZR - 1) / kZR;
+  FC_CHECK_ARG(nby <= 65535 && nbz <= 65535, "volume too large for the grid");
+  const size_t nwin = (size_t)nbx * nby * nbz * nv;
   void* ws = nullptr;
-  int rc = acquire_ws(device, SLOT_QUAD, bytes, stream, &ws);
+  int rc = acquire_ws(device, SLOT_QUAD, nwin * sizeof(uint2), stream, &ws);
   if (rc) return rc;
-  if (nv > 0) {
-    dim3 sg((QW + 255) / 256, QH, nv);
-    if (dt == FC_F32) stage_quads<FC_F32><<<sg, 256, 0, stream>>>(proj, nw, nh, norm, ws, QW, QH);
-    else if (dt == FC_BF16) stage_quads<FC_BF16><<<sg, 256, 0, stream>>>(proj, nw, nh, norm, ws, QW, QH);
-    else stage_quads<FC_F16><<<sg, 256, 0, stream>>>(proj, nw, nh, norm, ws, QW, QH);
-    FC_HIP(hipGetLastError());
-    BpArgs a;
-    a.vol = vol;
-    a.quad = ws;
-    a.views = views;
-    a.ox = origin[0]; a.oy = origin[1]; a.oz = origin[2];
-    a.nx = vs[0]; a.ny = vs[1]; a.nz = vs[2];
-    a.nw = nw; a.nh = nh; a.nviews = nv;
-    a.QW = QW; a.QH = QH;
-    dim3 grid((vs[0] + kBXY - 1) / kBXY, (vs[1] + kBXY - 1) / kBXY, (vs[2] + kZR - 1) / kZR);
-    FC_CHECK_ARG(grid.y <= 65535 && grid.z <= 65535, "volume too large for the grid");
-    int r2;
-    int tok = prof_begin("backproject", stream);
-    if (dt == FC_F32) r2 = dispatch_bp<FC_F32>(a, mode, grid, stream);
-    else if (dt == FC_BF16) r2 = dispatch_bp<FC_BF16>(a, mode, grid, stream);
-    else r2 = dispatch_bp<FC_F16>(a, mode, grid, stream);
-    prof_end(tok, stream);
-    if (r2) return r2;
-  }
+  WinArgs w;
+  w.views = views;
+  w.win = static_cast<uint2*>(ws);
+  w.ox = origin[0]; w.oy = origin[1]; w.oz = origin[2];
+  w.nx = vs[0]; w.ny = vs[1]; w.nz = vs[2];
+  w.nw = nw; w.nh = nh; w.nviews = nv;
+  w.nbx = nbx; w.nby = nby; w.nbz = nbz;
+  BpArgs a;
+  a.vol = vol;
+  a.proj = proj;
+  a.norm = norm;
+  a.views = views;
+  a.win = w.win;
+  a.ox = origin[0]; a.oy = origin[1]; a.oz = origin[2];
+  a.nx = vs[0]; a.ny = vs[1]; a.nz = vs[2];
+  a.nw = nw; a.nh = nh; a.nviews = nv;
+  a.nbx = nbx; a.nby = nby;
+  const dim3 grid(nbx, nby, nbz);
+  int tok = prof_begin("backproject", stream);
+  bp_windows<<<(unsigned)((nwin + 255) / 256), 256, 0, stream>>>(w);
+  int r2;
+  if (dt == FC_F32) r2 = dispatch_bp<FC_F32>(a, mode, grid, stream);
+  else if (dt == FC_BF16) r2 = dispatch_bp<FC_BF16>(a, mode, grid, stream);
+  else r2 = dispatch_bp<FC_F16>(a, mode, grid, stream);
+  prof_end(tok, stream);
+  if (r2) return r2;
   return release_ws(device, SLOT_QUAD, stream);
 }
 
@@ -262,82 +425,36 @@ int launch_stage_quads(const void* proj, int dt, int nw, int nh, int nv, const f
   return FC_OK;
 }
 
-// Ray-coherent ceiling for the backprojector: its fast-path inner loop (per column: one u and
-// weight per view, 16 z voxels whose rows advance by dv; clamp, bilinear quad gather, weighted
-// accumulate) over one 64 x 64 pixel patch per block that stays cache resident, with the
-// backprojector's lane layout and view-to-view drift. No per-view projective set-up and no fresh
-// projection data per view, so it bounds the voxel-view rate of the gather itself.
-// Bounds: u <= bu + 15.35, v <= bv + 4.85 + 15 * 1.3 = bv + 24.35, inside the 64 x 64 patch.
-template <int DT>
-__global__ __launch_bounds__(256) void gather_bench_bp_ray(const void* quad, int QW, int QH,
-                                                           int nviews, float* __restrict__ sink) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const float xl = (float)((wave & 1) * 8 + (lane & 7)), yl = (float)((wave >> 1) * 8 + (lane >> 3));
-  const int npx = (QW - 64) / 64, npy = (QH - 64) / 64;
-  const int b = blockIdx.x % (npx * npy);
-  const float bu = (float)((b % npx) * 64 + 2), bv = (float)((b / npx) * 64 + 2);
-  const float hmax = (float)(QH - 2);
-  float acc[16];
-#pragma unroll
-  for (int k = 0; k < 16; ++k) acc[k] = 0.0f;
-#pragma unroll 1
-  for (int t = 0; t < nviews; ++t) {
-    const float tt = (float)(t & 7);
-    const float u = bu + 0.7f * xl + 0.3f * yl + 0.05f * tt;
-    const float v0 = bv + 0.2f * xl + 0.1f * yl + 0.05f * tt;
-    const float w = 1.0f + 0.001f * tt;
-    const float fu = floorf(u);
-    const float au = u - fu;
-    const uint32_t iu = (uint32_t)(int)fu;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      float v = clampf(fmaf((float)k, 1.3f, v0), -1.0f, hmax);
-      float fv = floorf(v);
-      float av = v - fv;
-      uint32_t idx = (uint32_t)(int)fv * (uint32_t)QW + iu;
-      acc[k] = fmaf(w, bilinear_quad<DT>(quad, idx, au, av), acc[k]);
-    }
-  }
-  float sum = 0.0f;
-#pragma unroll
-  for (int k = 0; k < 16; ++k) sum += acc[k];
-  if (sum == 12345.678f) sink[0] = sum;
-}
-
 int launch_gather_bench_bp(int device, int dt, int ray, double* samples_per_s) {
   (void)device;
-  const int QW = 1024, QH = 1024;
-  size_t esz = (dt == FC_F32) ? 16 : 8;
-  void* quad = nullptr;
+  (void)dt;  // LDS windows hold FP32 whatever the storage type; only the staging converts
+  const int pitch = 1024, rows = 64 * 48;
+  std::vector<float> host((size_t)pitch * rows);
+  for (size_t q = 0; q < host.size(); ++q) host[q] = (float)((q * 2654435761u) % 1000) * 1e-3f;
+  float* src = nullptr;
   float* sink = nullptr;
-  FC_HIP(hipMalloc(&quad, (size_t)QW * QH * esz));
+  FC_HIP(hipMalloc(&src, host.size() * sizeof(float)));
   FC_HIP(hipMalloc(&sink, sizeof(float)));
-  FC_HIP(hipMemset(quad, 0x3c, (size_t)QW * QH * esz));
+  FC_HIP(hipMemcpy(src, host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice));
   hipEvent_t e0, e1;
   FC_HIP(hipEventCreate(&e0));
   FC_HIP(hipEventCreate(&e1));
   const int blocks = 256 * 16, nsamp = 4096;
   double best = 0.0;
-  for (int B : {8, 16, 32}) {
-    for (int rep = 0; rep < 2; ++rep) {
-      FC_HIP(hipEventRecord(e0, nullptr));
-      if (ray && dt == FC_F32) gather_bench_bp_ray<FC_F32><<<blocks, 256>>>(quad, QW, QH, nsamp / 16, sink);
-      else if (ray && dt == FC_BF16) gather_bench_bp_ray<FC_BF16><<<blocks, 256>>>(quad, QW, QH, nsamp / 16, sink);
-      else if (ray) gather_bench_bp_ray<FC_F16><<<blocks, 256>>>(quad, QW, QH, nsamp / 16, sink);
-      else if (dt == FC_F32) gather_bench_bp<FC_F32><<<blocks, 256>>>(quad, QW, QH, B, nsamp, sink);
-      else if (dt == FC_BF16) gather_bench_bp<FC_BF16><<<blocks, 256>>>(quad, QW, QH, B, nsamp, sink);
-      else gather_bench_bp<FC_F16><<<blocks, 256>>>(quad, QW, QH, B, nsamp, sink);
-      FC_HIP(hipEventRecord(e1, nullptr));
-      FC_HIP(hipEventSynchronize(e1));
-      float ms = 0.f;
-      FC_HIP(hipEventElapsedTime(&ms, e0, e1));
-      double rate = (double)blocks * 256.0 * nsamp / (ms * 1e-3);
-      if (rep == 1 && rate > best) best = rate;
-    }
+  for (int rep = 0; rep < 3; ++rep) {
+    FC_HIP(hipEventRecord(e0, nullptr));
+    if (ray) gather_bench_bp_ray<<<blocks, 256>>>(src, pitch, nsamp / kZR, sink);
+    else gather_bench_bp<<<blocks, 256>>>(src, pitch, nsamp, sink);
+    FC_HIP(hipEventRecord(e1, nullptr));
+    FC_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    FC_HIP(hipEventElapsedTime(&ms, e0, e1));
+    double rate = (double)blocks * 256.0 * nsamp / (ms * 1e-3);
+    if (rep > 0 && rate > best) best = rate;
   }
   FC_HIP(hipEventDestroy(e0));
   FC_HIP(hipEventDestroy(e1));
-  FC_HIP(hipFree(quad));
+  FC_HIP(hipFree(src));
   FC_HIP(hipFree(sink));
   *samples_per_s = best;
   return FC_OK;

@@ -213,3 +213,43 @@ def test_fbp_cosine_weight_closed_form():
     # centre column (j = Nw/2) and row i = zOrigin: weight 1
     assert abs(w[0, 2, 4] - 1.0) < 1e-15 and abs(w[1, 3, 4] - 1.0) < 1e-15
     assert abs(w[0, 0, 0] - 10.0 / math.sqrt(100 + 4 + 16)) < 1e-15
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_backproject_constant_projection_closed_form(mode):
+    # constant-1 data on a detector that every voxel projects well inside: the bilinear sample is
+    # 1, so each voxel receives the sum over views of its weight: U^2 (FDK), U (Katsevich) or 1,
+    # with U = D / (D + x cos t + y sin t) and x, y measured from the volume centre (index N // 2)
+    ang = np.linspace(0.3, 0.3 + 2 * np.pi, 7, endpoint=False)
+    D, nz, ny, nx = 40.0, 6, 9, 11
+    prob = oracle.Problem(ang, np.full(7, nz / 2.0), D, 1.0, vol_dim=(nx, ny, nz),
+                          proj_dim=(64, 64))
+    v = oracle.backproject(prob, np.ones((7, 64, 64)), (nz, ny, nx), mode=mode, norm=np.ones(7))
+    x = np.arange(nx) - nx // 2
+    y = np.arange(ny) - ny // 2
+    U = D / (D + x[None, None, :] * np.cos(ang)[:, None, None]
+             + y[None, :, None] * np.sin(ang)[:, None, None])
+    expect = {0: np.full(U.shape, 1.0), 1: U, 2: U ** 2}[mode].sum(0)
+    assert np.allclose(v, np.broadcast_to(expect, v.shape), rtol=1e-12, atol=1e-12)
+
+
+def test_backproject_single_pixel_delta():
+    # one view, a unit delta at detector pixel (w0, h0): voxel (x, y, z) receives U^2 times the
+    # bilinear hat weight (1 - |w - w0|)(1 - |h - h0|) of its projected position
+    ang = np.array([0.37])
+    D, nz, ny, nx, nw, nh = 30.0, 8, 10, 12, 24, 20
+    prob = oracle.Problem(ang, np.array([nz / 2.0]), D, 1.0, vol_dim=(nx, ny, nz),
+                          proj_dim=(nw, nh))
+    w0, h0 = 13, 9
+    p = np.zeros((1, nh, nw))
+    p[0, h0, w0] = 1.0
+    v = oracle.backproject(prob, p, (nz, ny, nx), mode=2, norm=np.ones(1))
+    z, y, x = np.meshgrid(np.arange(nz) - nz // 2, np.arange(ny) - ny // 2,
+                          np.arange(nx) - nx // 2, indexing="ij")
+    c, s = math.cos(ang[0]), math.sin(ang[0])
+    U = D / (D + x * c + y * s)
+    w = (y * c - x * s) * U + nw // 2
+    h = (z - (nz / 2.0 - nz // 2)) * U + nh // 2
+    hat = np.clip(1 - np.abs(w - w0), 0, None) * np.clip(1 - np.abs(h - h0), 0, None)
+    assert np.count_nonzero(hat) > 4
+    assert np.allclose(v, U ** 2 * hat, rtol=1e-10, atol=1e-12)
