@@ -28,7 +28,7 @@ namespace fc {
 namespace {
 
 constexpr int kTile = 16;     // 16x16 pixel tile per 256-thread block, 8x8 per wave
-constexpr int kLdsCap = 8192;  // projector LDS brick budget, floats per block (32 KiB)
+constexpr int kLdsCap = 6144;  // projector LDS brick budget, floats per block (24 KiB)
 
 typedef float f32x2_a4 __attribute__((ext_vector_type(2), aligned(4)));
 
@@ -520,88 +520,152 @@ __device__ __forceinline__ void scatter_global(float* accp, uint32_t PX, uint32_
   v = w11 * ax; unsafeAtomicAdd(p + PXY + PX + 1, v); unsafeAtomicAdd(p + PXY + PX, w11 - v);
 }
 
-// LDS-staged exact transpose of fp_lds_kernel: the same tiles, slab plans and per-slab k ranges;
-// each thread scatters its samples' weights times its pixel value into the slab's LDS brick
-// (ds_add_f32), and the brick is then added to the padded global accumulator with one atomic per
-// nonzero element (bricks of different tiles overlap).
-template <int DT, int S>
-__global__ __launch_bounds__(256) void adjoint_lds_kernel(AdjArgs a) {
-  constexpr int DP = S + 3;
-  __shared__ float brick[kLdsCap];
-  int t, j0, i0;
-  fp_tile_origin(a.tiles_w, a.tiles_per_view, a.group, a.nviews, &t, &j0, &i0);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int j = j0 + (wave & 1) * 8 + (lane & 7);
-  const int i = i0 + (wave >> 1) * 8 + (lane >> 3);
-  const bool pix = (t < a.nviews) && (j < a.nw) && (i < a.nh);
-  const double* V = a.views + (size_t)min(t, a.nviews - 1) * FC_VIEW_DOUBLES;
-  float y = 0.0f;
-  if (pix) {
-    size_t pidx = ((size_t)t * a.nh + i) * a.nw + j;
-    y = load_elem<DT>(a.proj, pidx) * (a.norm ? a.norm[t] : 1.0f) * (float)a.h;
+// Voxel-owned exact transpose of the projector. A block owns the 16^3 "base cells" of a brick
+// (voxel indices [16 b - 1, 16 b + 15) per axis, which tile [-1, n - 1]): it walks every ray of
+// every view whose detector window (adj_windows) can reach the brick, recomputes the ray's samples
+// exactly as the projector does (same ray_segment, same FMA positions), keeps those whose base
+// corner floor(q) lies in the brick — so every sample of every ray is scattered by exactly one
+// block — and adds y h norm times its eight trilinear weights into a 17^3 LDS accumulator
+// (ds_add_f32). The accumulator is added to the volume once at the end (one global atomic per
+// nonzero element; neighbouring bricks share one face). (ray, view) items are dealt to the
+// threads round-robin across views, so lanes stay busy when windows are small.
+constexpr int kAB = 16;             // base cells per brick edge
+constexpr int kAL = kAB + 1;        // accumulator edge (corners reach one cell further)
+constexpr int kAL3 = kAL * kAL * kAL;
+
+struct AdjVArgs {
+  float* vol;          // unpadded output block, accumulated
+  const void* proj;
+  const double* views;
+  const float* norm;
+  const uint2* win;    // per (brick, view): x = j0 | i0 << 16, y = Wj | Wi << 16
+  double ox, oy, oz;
+  double h;
+  int nx, ny, nz;
+  int nw, nh, nviews;
+  int nbx, nby, nbz;
+  int max_steps;
+};
+
+// One thread per (brick, view): the detector pixels whose rays can carry samples with a base cell
+// in the brick — the projective image of the brick's box [16 b - 1, 16 b + 15]^3 (voxel
+// coordinates), widened by one pixel for FP32 sample rounding and clipped to the detector. A box
+// not wholly in front of the source, or a cylindrical detector, gets the whole detector.
+__global__ __launch_bounds__(256) void adj_windows(AdjVArgs a, uint2* __restrict__ out) {
+  const size_t total = (size_t)a.nbx * a.nby * a.nbz * a.nviews;
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= total) return;
+  const int t = (int)(idx % (size_t)a.nviews);
+  const size_t b = idx / (size_t)a.nviews;
+  const int bx = (int)(b % a.nbx), by = (int)((b / a.nbx) % a.nby),
+            bz = (int)(b / ((size_t)a.nbx * a.nby));
+  const double* V = a.views + (size_t)t * FC_VIEW_DOUBLES;
+  const double* M = V + 12;
+  const double xs[2] = {kAB * bx - 1 + a.ox, kAB * bx + kAB - 1 + a.ox};
+  const double ys[2] = {kAB * by - 1 + a.oy, kAB * by + kAB - 1 + a.oy};
+  const double zs[2] = {kAB * bz - 1 + a.oz, kAB * bz + kAB - 1 + a.oz};
+  double umin = 1e300, umax = -1e300, vmin = 1e300, vmax = -1e300;
+  bool ok = V[24] != (double)FC_DET_CYLINDER;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const double X = xs[c & 1], Y = ys[(c >> 1) & 1], Z = zs[c >> 2];
+    const double C = M[8] * X + M[9] * Y + M[10] * Z + M[11];
+    ok = ok && C > 0.0;
+    const double u = (M[0] * X + M[1] * Y + M[2] * Z + M[3]) / C;
+    const double v = (M[4] * X + M[5] * Y + M[6] * Z + M[7]) / C;
+    umin = fmin(umin, u); umax = fmax(umax, u);
+    vmin = fmin(vmin, v); vmax = fmax(vmax, v);
   }
-  RaySeg r;
-  r.nk = 0;
-  r.px = r.py = r.pz = r.sx = r.sy = r.sz = 0.f;
-  if (pix) r = ray_segment(V, j, i, a.ox, a.oy, a.oz, a.nx, a.ny, a.nz, a.h, a.max_steps, 2.0f);
-  if (y == 0.0f) r.nk = 0;
-  SlabPlan P;
-  make_plan<S>(P, V, r, j0, i0, a.nw, a.nh, a.ox, a.oy, a.oz, a.PX, a.PY, a.nz);
-  const bool bad = (r.nk > 0 && !(P.sd * P.dir > 0.2f * (float)a.h)) || !P.corners_ok;
-  const bool fallback = __syncthreads_or(bad);
-  const uint32_t PXu = (uint32_t)a.PX, PXY = (uint32_t)a.PX * (uint32_t)a.PY;
-  if (fallback) {
-    for (int k = 0; k < r.nk; ++k) {
-      float kf = (float)k;
-      scatter_global(a.acc, PXu, PXY, fmaf(kf, r.sx, r.px), fmaf(kf, r.sy, r.py),
-                     fmaf(kf, r.sz, r.pz), y);
-    }
-    return;
+  int j0 = 0, j1 = a.nw - 1, i0 = 0, i1 = a.nh - 1;
+  if (ok && umin > -1e9 && umax < 1e9 && vmin > -1e9 && vmax < 1e9) {
+    j0 = max(j0, (int)ceil(umin) - 1); j1 = min(j1, (int)floor(umax) + 1);
+    i0 = max(i0, (int)ceil(vmin) - 1); i1 = min(i1, (int)floor(vmax) + 1);
   }
-  for (int m = 0; m < P.nslabs; ++m) {
-    const SlabBox B = slab_box<S>(P, m);
-    if (B.E1 <= 0 || B.E2 <= 0) continue;
-    const int PS = DP * B.E1;
-    const int NE = PS * B.E2;
-    if (NE > a.lds_cap) {
-      for (int k = B.kin; k < B.kout; ++k) {
-        float kf = (float)k;
-        scatter_global(a.acc, PXu, PXY, fmaf(kf, r.sx, r.px), fmaf(kf, r.sy, r.py),
-                       fmaf(kf, r.sz, r.pz), y);
+  uint2 w = make_uint2(0u, 0u);
+  if (j1 >= j0 && i1 >= i0) {
+    w.x = (uint32_t)j0 | ((uint32_t)i0 << 16);
+    w.y = (uint32_t)(j1 - j0 + 1) | ((uint32_t)(i1 - i0 + 1) << 16);
+  }
+  out[idx] = w;
+}
+
+template <int DT>
+__global__ __launch_bounds__(256) void adjoint_vox_kernel(AdjVArgs a) {
+  __shared__ float acc[kAL3];
+  for (int e = threadIdx.x; e < kAL3; e += 256) acc[e] = 0.0f;
+  __syncthreads();
+  const int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+  // base-cell range in the padded sample frame (voxel index + 2)
+  const int lox = kAB * bx + 1, loy = kAB * by + 1, loz = kAB * bz + 1;
+  const float flx = (float)lox, fly = (float)loy, flz = (float)loz;
+  const uint2* W = a.win + ((size_t)(bz * a.nby + by) * a.nbx + bx) * (size_t)a.nviews;
+  int t = 0, start = 0;
+  uint2 w = a.nviews > 0 ? W[0] : make_uint2(0u, 0u);
+  for (int g = threadIdx.x;; g += 256) {
+    int Wj = (int)(w.y & 0xffffu), npix = Wj * (int)(w.y >> 16);
+    while (t < a.nviews && g >= start + npix) {
+      start += npix;
+      if (++t < a.nviews) {
+        w = W[t];
+        Wj = (int)(w.y & 0xffffu);
+        npix = Wj * (int)(w.y >> 16);
       }
-      continue;
     }
-    __syncthreads();  // the previous brick has been flushed
-    for (int e = threadIdx.x; e < NE; e += 256) brick[e] = 0.0f;
-    __syncthreads();
-    const uint32_t E1u = (uint32_t)B.E1;
-    const uint32_t boff = (uint32_t)(B.dlo + DP * (B.e1lo + B.E1 * B.e2lo));
-    const uint32_t zfl = (uint32_t)PS;
-    float* base = brick - boff;
-    float kf = (float)B.kin;
-    for (int k = B.kin; k < B.kout; ++k, kf += 1.0f) {
-      const float qd = fmaf(kf, P.sd, P.pd0), q1 = fmaf(kf, P.s1, P.p10),
-                  q2 = fmaf(kf, P.s2, P.p20);
-      const uint32_t id = (uint32_t)(int)qd, i1 = (uint32_t)(int)q1, i2 = (uint32_t)(int)q2;
-      const float ad = __builtin_amdgcn_fractf(qd), a1f = __builtin_amdgcn_fractf(q1),
-                  a2f = __builtin_amdgcn_fractf(q2);
-      float* L0 = base + mad_u24(mad_u24(i2, E1u, i1), (uint32_t)DP, id);
-      float* L1 = L0 + zfl;
-      const float wz1 = y * a2f, wz0 = y - wz1;
-      const float w10 = wz0 * a1f, w00 = wz0 - w10, w11 = wz1 * a1f, w01 = wz1 - w11;
+    if (t >= a.nviews) break;
+    const int p = g - start;
+    const int i = (int)(w.x >> 16) + p / Wj, j = (int)(w.x & 0xffffu) + p % Wj;
+    float y = load_elem<DT>(a.proj, ((size_t)t * a.nh + i) * a.nw + j) *
+              (a.norm ? a.norm[t] : 1.0f) * (float)a.h;
+    if (y == 0.0f) continue;
+    const double* V = a.views + (size_t)t * FC_VIEW_DOUBLES;
+    const RaySeg r = ray_segment(V, j, i, a.ox, a.oy, a.oz, a.nx, a.ny, a.nz, a.h, a.max_steps,
+                                 2.0f);
+    if (r.nk == 0) continue;
+    // k range of samples whose base cell can lie in the brick: per axis, q in [lo, lo + 16);
+    // widened by one step for FP32 rounding, then every sample is checked exactly
+    float klo = 0.0f, khi = (float)(r.nk - 1);
+    auto clip = [&](float p0, float st, float lo) {
+      if (st != 0.0f) {
+        const float ka = (lo - p0) / st, kb = (lo + (float)kAB - p0) / st;
+        klo = fmaxf(klo, fminf(ka, kb) - 1.0f);
+        khi = fminf(khi, fmaxf(ka, kb) + 1.0f);
+      } else if (!(p0 >= lo - 1.0f && p0 < lo + (float)kAB + 1.0f)) {
+        khi = -1.0f;
+      }
+    };
+    clip(r.px, r.sx, flx);
+    clip(r.py, r.sy, fly);
+    clip(r.pz, r.sz, flz);
+    if (!(khi >= klo)) continue;
+    const int k0 = max(0, (int)ceilf(klo)), k1 = min(r.nk - 1, (int)floorf(khi));
+    for (int k = k0; k <= k1; ++k) {
+      const float kf = (float)k;
+      const float qx = fmaf(kf, r.sx, r.px), qy = fmaf(kf, r.sy, r.py), qz = fmaf(kf, r.sz, r.pz);
+      const uint32_t cx = (uint32_t)((int)qx - lox), cy = (uint32_t)((int)qy - loy),
+                     cz = (uint32_t)((int)qz - loz);
+      if (cx >= (uint32_t)kAB || cy >= (uint32_t)kAB || cz >= (uint32_t)kAB) continue;
+      const float ax = __builtin_amdgcn_fractf(qx), ay = __builtin_amdgcn_fractf(qy),
+                  az = __builtin_amdgcn_fractf(qz);
+      float* L = acc + (cz * kAL + cy) * kAL + cx;
+      const float wz1 = y * az, wz0 = y - wz1;
+      const float w10 = wz0 * ay, w00 = wz0 - w10, w11 = wz1 * ay, w01 = wz1 - w11;
       float v;
-      v = w00 * ad; atomicAdd(L0 + 1, v); atomicAdd(L0, w00 - v);
-      v = w10 * ad; atomicAdd(L0 + DP + 1, v); atomicAdd(L0 + DP, w10 - v);
-      v = w01 * ad; atomicAdd(L1 + 1, v); atomicAdd(L1, w01 - v);
-      v = w11 * ad; atomicAdd(L1 + DP + 1, v); atomicAdd(L1 + DP, w11 - v);
+      v = w00 * ax; atomicAdd(L + 1, v); atomicAdd(L, w00 - v);
+      v = w10 * ax; atomicAdd(L + kAL + 1, v); atomicAdd(L + kAL, w10 - v);
+      v = w01 * ax; atomicAdd(L + kAL * kAL + 1, v); atomicAdd(L + kAL * kAL, w01 - v);
+      v = w11 * ax; atomicAdd(L + kAL * kAL + kAL + 1, v); atomicAdd(L + kAL * kAL + kAL, w11 - v);
     }
-    __syncthreads();
-    const float invE1 = 1.0f / (float)B.E1;
-    for (int e = threadIdx.x; e < NE; e += 256) {
-      const float v = brick[e];
-      if (v != 0.0f && (int)((uint32_t)e % DP) <= P.PD - 1 - B.dlo)
-        unsafeAtomicAdd(a.acc + brick_elem_offset<DP>(P, B, (uint32_t)e, invE1), v);
-    }
+  }
+  __syncthreads();
+  // LDS element (ex, ey, ez) is voxel (16 b - 1 + e) of the block; voxels outside [0, n) are the
+  // projector's zero halo
+  for (int e = threadIdx.x; e < kAL3; e += 256) {
+    const float v = acc[e];
+    if (v == 0.0f) continue;
+    const int ex = e % kAL, ey = (e / kAL) % kAL, ez = e / (kAL * kAL);
+    const int x = kAB * bx - 1 + ex, yy = kAB * by - 1 + ey, z = kAB * bz - 1 + ez;
+    if (x < 0 || yy < 0 || z < 0 || x >= a.nx || yy >= a.ny || z >= a.nz) continue;
+    unsafeAtomicAdd(a.vol + ((size_t)z * a.ny + yy) * a.nx + x, v);
   }
 }
 
@@ -897,46 +961,70 @@ int launch_adjoint(int device, float* vol, const int vs[3], const double origin[
   const int PX = nx + 4, PY = ny + 4, PZ = nz + 4;
   FC_CHECK_ARG(PY <= 65535 && PZ <= 65535, "volume dimension too large");
   FC_CHECK_ARG((size_t)PX * PY * PZ < (1ull << 32), "volume too large for 32-bit voxel offsets");
-  size_t n = (size_t)PX * PY * PZ;
-  void* ws = nullptr;
-  int rc = acquire_ws(device, SLOT_OCTET, n * sizeof(float), stream, &ws);
-  if (rc) return rc;
-  float* acc = static_cast<float*>(ws);
-  zero_fill<<<(unsigned)std::min<size_t>((n + 255) / 256, 65536), 256, 0, stream>>>(acc, n);
-  FC_HIP(hipGetLastError());
-  AdjArgs a;
-  a.acc = acc;
+  FC_CHECK_ARG(ps[0] <= 65535 && ps[1] <= 65535, "detector too large for the adjoint windows");
+  if (ps[2] == 0 || nx == 0 || ny == 0 || nz == 0) return FC_OK;
+  if (fp_layout_choice() != FP_LDS) {
+    // global-atomic scatter into a padded accumulator (reference path for tests)
+    size_t n = (size_t)PX * PY * PZ;
+    void* ws = nullptr;
+    int rc = acquire_ws(device, SLOT_OCTET, n * sizeof(float), stream, &ws);
+    if (rc) return rc;
+    float* acc = static_cast<float*>(ws);
+    zero_fill<<<(unsigned)std::min<size_t>((n + 255) / 256, 65536), 256, 0, stream>>>(acc, n);
+    FC_HIP(hipGetLastError());
+    AdjArgs a;
+    a.acc = acc;
+    a.proj = proj;
+    a.views = views;
+    a.norm = norm;
+    a.ox = origin[0]; a.oy = origin[1]; a.oz = origin[2];
+    a.h = h;
+    a.nx = nx; a.ny = ny; a.nz = nz;
+    a.PX = PX; a.PY = PY;
+    a.nw = ps[0]; a.nh = ps[1]; a.nviews = ps[2];
+    a.tiles_w = (a.nw + kTile - 1) / kTile;
+    a.tiles_per_view = a.tiles_w * ((a.nh + kTile - 1) / kTile);
+    a.group = std::min(8, std::max(1, a.nviews));
+    a.max_steps = max_steps_for(vs, h);
+    long long nblocks = (long long)a.tiles_per_view * a.nviews;
+    FC_CHECK_ARG(nblocks < (1ll << 31), "too many projection tiles");
+    int tok = prof_begin("adjoint", stream);
+    if (dt == FC_F32) adjoint_kernel<FC_F32><<<(int)nblocks, 256, 0, stream>>>(a);
+    else if (dt == FC_BF16) adjoint_kernel<FC_BF16><<<(int)nblocks, 256, 0, stream>>>(a);
+    else adjoint_kernel<FC_F16><<<(int)nblocks, 256, 0, stream>>>(a);
+    prof_end(tok, stream);
+    FC_HIP(hipGetLastError());
+    dim3 ug((nx + 255) / 256, ny, nz);
+    unpad_add<<<ug, 256, 0, stream>>>(acc, PX, PY, vol, nx, ny, nz);
+    FC_HIP(hipGetLastError());
+    return release_ws(device, SLOT_OCTET, stream);
+  }
+  AdjVArgs a;
+  a.vol = vol;
   a.proj = proj;
   a.views = views;
   a.norm = norm;
   a.ox = origin[0]; a.oy = origin[1]; a.oz = origin[2];
   a.h = h;
   a.nx = nx; a.ny = ny; a.nz = nz;
-  a.PX = PX; a.PY = PY;
   a.nw = ps[0]; a.nh = ps[1]; a.nviews = ps[2];
-  a.tiles_w = (a.nw + kTile - 1) / kTile;
-  a.tiles_per_view = a.tiles_w * ((a.nh + kTile - 1) / kTile);
-  a.group = std::min(8, std::max(1, a.nviews));
+  // bricks of base cells tile [-1, n - 1]
+  a.nbx = (nx + 1 + kAB - 1) / kAB;
+  a.nby = (ny + 1 + kAB - 1) / kAB;
+  a.nbz = (nz + 1 + kAB - 1) / kAB;
   a.max_steps = max_steps_for(vs, h);
-  const char* cap = std::getenv("FASTCONE_FP_LDS_CAP");
-  a.lds_cap = cap ? std::max(0, std::min(kLdsCap, std::atoi(cap))) : kLdsCap;
-  const bool lds = fp_layout_choice() == FP_LDS;
-  long long nblocks = (long long)a.tiles_per_view * a.nviews;
-  FC_CHECK_ARG(nblocks < (1ll << 31), "too many projection tiles");
-  if (nblocks > 0) {
-    int tok = prof_begin("adjoint", stream);
-    if (lds) {
-      if (dt == FC_F32) adjoint_lds_kernel<FC_F32, 8><<<(int)nblocks, 256, 0, stream>>>(a);
-      else if (dt == FC_BF16) adjoint_lds_kernel<FC_BF16, 8><<<(int)nblocks, 256, 0, stream>>>(a);
-      else adjoint_lds_kernel<FC_F16, 8><<<(int)nblocks, 256, 0, stream>>>(a);
-    } else if (dt == FC_F32) adjoint_kernel<FC_F32><<<(int)nblocks, 256, 0, stream>>>(a);
-    else if (dt == FC_BF16) adjoint_kernel<FC_BF16><<<(int)nblocks, 256, 0, stream>>>(a);
-    else adjoint_kernel<FC_F16><<<(int)nblocks, 256, 0, stream>>>(a);
-    prof_end(tok, stream);
-    FC_HIP(hipGetLastError());
-  }
-  dim3 ug((nx + 255) / 256, ny, nz);
-  unpad_add<<<ug, 256, 0, stream>>>(acc, PX, PY, vol, nx, ny, nz);
+  const size_t nwin = (size_t)a.nbx * a.nby * a.nbz * a.nviews;
+  void* ws = nullptr;
+  int rc = acquire_ws(device, SLOT_OCTET, nwin * sizeof(uint2), stream, &ws);
+  if (rc) return rc;
+  a.win = static_cast<uint2*>(ws);
+  int tok = prof_begin("adjoint", stream);
+  adj_windows<<<(unsigned)((nwin + 255) / 256), 256, 0, stream>>>(a, static_cast<uint2*>(ws));
+  const dim3 grid(a.nbx, a.nby, a.nbz);
+  if (dt == FC_F32) adjoint_vox_kernel<FC_F32><<<grid, 256, 0, stream>>>(a);
+  else if (dt == FC_BF16) adjoint_vox_kernel<FC_BF16><<<grid, 256, 0, stream>>>(a);
+  else adjoint_vox_kernel<FC_F16><<<grid, 256, 0, stream>>>(a);
+  prof_end(tok, stream);
   FC_HIP(hipGetLastError());
   return release_ws(device, SLOT_OCTET, stream);
 }
@@ -989,11 +1077,11 @@ int launch_gather_bench_fp(int device, int dt, int plain, double* samples_per_s)
   for (int B : {4, 8, 16}) {
     for (int rep = 0; rep < 2; ++rep) {
       FC_HIP(hipEventRecord(e0, nullptr));
-      // LDS bricks of 11 x E x E with E = 12, 18, 24 (the projector's typical slab bricks;
-      // 11 * 24^2 floats fit the LDS budget)
+      // LDS bricks of 11 x E x E with E = 12, 18, 23 (the projector's typical slab bricks;
+      // 11 * 23^2 floats fit the LDS budget)
       if (ray && B == 4) gather_bench_lds_ray<8, 2><<<blocks, 256>>>(a, nsamp / 16, sink);
       else if (ray) gather_bench_lds_ray<8, 4><<<blocks, 256>>>(a, nsamp / 16, sink);
-      else if (lds) gather_bench_lds<8><<<blocks, 256>>>(a, B == 4 ? 12 : (B == 8 ? 18 : 24), nsamp, sink);
+      else if (lds) gather_bench_lds<8><<<blocks, 256>>>(a, B == 4 ? 12 : (B == 8 ? 18 : 23), nsamp, sink);
       else if (layout == FP_PLAIN) gather_bench_fp<FC_F32, FP_PLAIN><<<blocks, 256>>>(a, B, nsamp, sink);
       else if (dt == FC_F32) gather_bench_fp<FC_F32, FP_OCTET><<<blocks, 256>>>(a, B, nsamp, sink);
       else if (dt == FC_BF16) gather_bench_fp<FC_BF16, FP_OCTET><<<blocks, 256>>>(a, B, nsamp, sink);

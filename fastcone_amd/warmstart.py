@@ -11,6 +11,11 @@ plain FDK-start CGLS is used (fallback). Wall time of the warm start includes FD
 
     python -m fastcone_amd.warmstart train --pairs 2000 --epochs 10 --out unet.npz
     python -m fastcone_amd.warmstart eval --model unet.npz
+
+Training can be streamed over several runs: `--first-pair S` selects the phantom seeds
+S .. S + pairs - 1 and `--resume CKPT` continues from the weights and Adam state that a previous
+run saved next to its `--out` file (CKPT.npz + CKPT.adam.pt), so a long schedule of fresh pairs
+can be split into runs of bounded length.
 """
 import argparse
 import json
@@ -65,25 +70,36 @@ def make_pair(seed, geom, device, cgls_iters=200):
     return x_fdk, x_cg, vol
 
 
-def generate(npairs, device, cgls_iters=200, log=print):
+def generate(npairs, device, cgls_iters=200, log=print, first=0, time_budget=None):
+    """Pairs for phantom seeds first .. first + npairs - 1 (BF16 on the device); stops early once
+    time_budget seconds are spent and returns the pairs made so far."""
     geom = train_geometry()
     n = geom.vol_shape
     X = torch.empty((npairs,) + n, dtype=torch.bfloat16, device=device)
     Y = torch.empty((npairs,) + n, dtype=torch.bfloat16, device=device)
     t0 = time.perf_counter()
+    made = 0
     for s in range(npairs):
-        xf, xc, _ = make_pair(s, geom, device, cgls_iters)
+        xf, xc, _ = make_pair(first + s, geom, device, cgls_iters)
         X[s] = xf.to(torch.bfloat16)
         Y[s] = xc.to(torch.bfloat16)
-        if s % 50 == 0 or s == npairs - 1:
+        made = s + 1
+        if s % 25 == 0 or s == npairs - 1:
             torch.cuda.synchronize()
             log(f"pairs {s + 1}/{npairs} ({time.perf_counter() - t0:.1f} s)")
-    return X, Y
+        if time_budget is not None and time.perf_counter() - t0 > time_budget:
+            torch.cuda.synchronize()
+            log(f"generation time budget reached after {made} pairs")
+            break
+    return X[:made], Y[:made]
 
 
-def train(X, Y, epochs, batch=4, lr=1e-3, seed=0, log=print, val_frac=0.05):
+def train(X, Y, epochs, batch=4, lr=1e-3, seed=0, log=print, val_frac=0.05, resume=None):
     net = UNet3D(seed=seed).to(X.device)
     opt = torch.optim.Adam(net.parameters(), lr=lr)
+    if resume:
+        net.load(resume + ".npz")
+        opt.load_state_dict(torch.load(resume + ".adam.pt", weights_only=True))
     n = X.shape[0]
     nval = max(1, int(round(n * val_frac)))
     tr = np.arange(n - nval)
@@ -111,7 +127,7 @@ def train(X, Y, epochs, batch=4, lr=1e-3, seed=0, log=print, val_frac=0.05):
         hist.append({"epoch": ep, "train_loss": tot / max(1, len(tr) // batch),
                      "val_mse_vs_fdk_mse": val, "s": time.perf_counter() - t0})
         log(json.dumps(hist[-1]))
-    return net, hist
+    return net, hist, opt
 
 
 def time_to_residual(geom, b, x0, tol=1e-3, max_iters=400, atb_norm=None):
@@ -173,6 +189,12 @@ def main(argv=None):
     ap.add_argument("--model", default=None)
     ap.add_argument("--out", default="unet_warmstart.npz")
     ap.add_argument("--seed", type=int, default=777)
+    ap.add_argument("--first-pair", type=int, default=0, help="first phantom seed of this run")
+    ap.add_argument("--resume", default=None,
+                    help="checkpoint prefix (PREFIX.npz + PREFIX.adam.pt) to continue from")
+    ap.add_argument("--time-budget", type=float, default=None,
+                    help="seconds of pair generation after which training starts anyway")
+    ap.add_argument("--no-eval", action="store_true")
     a = ap.parse_args(argv)
     dev = torch.device("cuda", 0)
     log = lambda s: print(s, flush=True)
@@ -185,11 +207,17 @@ def main(argv=None):
                         "cgls_err": float((xc - vol).norm() / vol.norm())}))
         return
     if a.cmd == "train":
-        X, Y = generate(a.pairs, dev, a.cgls_iters, log)
-        net, hist = train(X, Y, a.epochs, log=log)
-        net.save(a.out)
+        X, Y = generate(a.pairs, dev, a.cgls_iters, log, first=a.first_pair,
+                        time_budget=a.time_budget)
+        net, hist, opt = train(X, Y, a.epochs, log=log, resume=a.resume)
+        prefix = a.out[:-4] if a.out.endswith(".npz") else a.out
+        net.save(prefix + ".npz")
+        torch.save(opt.state_dict(), prefix + ".adam.pt")
+        log(json.dumps({"pairs": int(X.shape[0]), "first_pair": a.first_pair,
+                        "resumed_from": a.resume}))
         del X, Y
-        evaluate(net, dev, a.seed, log=log)
+        if not a.no_eval:
+            evaluate(net, dev, a.seed, log=log)
         return
     net = UNet3D().to(dev)
     if a.model:

@@ -152,13 +152,11 @@ def test_backproject_accumulates_into_out(gpu):
     assert rel(out.cpu().numpy() - base, ref) < 5e-5
 
 
-@pytest.mark.parametrize("layout,cap", [("lds", None), ("lds", "600"), ("lds", "0"),
-                                        ("plain", None)])
-def test_adjoint_paths_agree(gpu, layout, cap, monkeypatch):
+@pytest.mark.parametrize("layout", ["lds", "plain"])
+def test_adjoint_paths_agree(gpu, layout, monkeypatch):
+    # lds: voxel-owned bricks with LDS accumulation (default); plain: global-atomic scatter
     from fastcone_amd import ops
     monkeypatch.setenv("FASTCONE_FP_LAYOUT", layout)
-    if cap is not None:
-        monkeypatch.setenv("FASTCONE_FP_LDS_CAP", cap)
     rng = np.random.default_rng(32)
     g, prob = _setup(rng)
     y = rng.standard_normal(g.proj_shape).astype(np.float32)

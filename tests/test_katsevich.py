@@ -102,6 +102,25 @@ def test_helical_projector_parity(gpu):
     assert _rel(p, ref) < 5e-5
 
 
+
+@pytest.mark.gpu
+def test_helical_adjoint_dot_product(gpu):
+    # cylindrical records take the adjoint's whole-detector windows; the pair must stay matched
+    import torch
+    from fastcone_amd import ops
+    from fastcone_amd.katsevich import HelixScan
+    hx = small_helix()
+    scan = HelixScan(hx.n, hx.dl, hx.lambdas[0], hx.zs[0], HP, R, SDD, DG, DROW, NW, NH)
+    gs = scan.geometry((N, N, N)).subset(slice(0, hx.n, 53))
+    rng = np.random.default_rng(3)
+    x = torch.from_numpy(rng.standard_normal((N, N, N)).astype(np.float32)).to(gpu)
+    y = torch.from_numpy(rng.standard_normal(gs.proj_shape).astype(np.float32)).to(gpu)
+    ax = ops.project(x, gs, step=0.5)
+    aty = ops.adjoint(y, gs, step=0.5)
+    lhs = float((ax.double() * y.double()).sum())
+    rhs = float((x.double() * aty.double()).sum())
+    assert abs(lhs - rhs) / abs(lhs) < 1e-5
+
 @pytest.mark.gpu
 def test_katsevich_parity(gpu):
     import torch
