@@ -549,6 +549,7 @@ struct AdjVArgs {
   int nx, ny, nz;
   int nw, nh, nviews;
   int nbx, nby, nbz;
+  int nsplit;          // view ranges per brick (grid z = nbz * nsplit)
   int max_steps;
 };
 
@@ -625,24 +626,28 @@ __global__ __launch_bounds__(256) void adjoint_vox_kernel(AdjVArgs a) {
   if (!(ymax > 0.0f) || !(ymax < 3.0e38f)) return;  // all-zero data (uniform exit)
   const float fscale = 1073741824.0f / ymax;        // 2^30 / max|y|
   __syncthreads();
-  const int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+  const int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z % a.nbz;
+  // this block's share of the views (small volumes split the views over several blocks)
+  const int split = blockIdx.z / a.nbz;
+  const int tbeg = (int)((long long)a.nviews * split / a.nsplit);
+  const int tend = (int)((long long)a.nviews * (split + 1) / a.nsplit);
   // base-cell range in the padded sample frame (voxel index + 2)
   const int lox = kAB * bx + 1, loy = kAB * by + 1, loz = kAB * bz + 1;
   const float flx = (float)lox, fly = (float)loy, flz = (float)loz;
   const uint2* W = a.win + ((size_t)(bz * a.nby + by) * a.nbx + bx) * (size_t)a.nviews;
-  int t = 0, start = 0;
-  uint2 w = a.nviews > 0 ? W[0] : make_uint2(0u, 0u);
+  int t = tbeg, start = 0;
+  uint2 w = tbeg < tend ? W[tbeg] : make_uint2(0u, 0u);
   for (int g = threadIdx.x;; g += 256) {
     int Wj = (int)(w.y & 0xffffu), npix = Wj * (int)(w.y >> 16);
-    while (t < a.nviews && g >= start + npix) {
+    while (t < tend && g >= start + npix) {
       start += npix;
-      if (++t < a.nviews) {
+      if (++t < tend) {
         w = W[t];
         Wj = (int)(w.y & 0xffffu);
         npix = Wj * (int)(w.y >> 16);
       }
     }
-    if (t >= a.nviews) break;
+    if (t >= tend) break;
     const int p = g - start;
     const int i = (int)(w.x >> 16) + p / Wj, j = (int)(w.x & 0xffffu) + p % Wj;
     const float y = load_elem<DT>(a.proj, ((size_t)t * a.nh + i) * a.nw + j) *
@@ -1050,6 +1055,10 @@ int launch_adjoint(int device, float* vol, const int vs[3], const double origin[
   a.nby = (ny + 1 + kAB - 1) / kAB;
   a.nbz = (nz + 1 + kAB - 1) / kAB;
   a.max_steps = max_steps_for(vs, h);
+  // at least ~8 blocks per CU: split the views of small volumes over several blocks per brick
+  const long long nbricks = (long long)a.nbx * a.nby * a.nbz;
+  a.nsplit = (int)std::max(1LL, std::min<long long>((2048 + nbricks - 1) / nbricks, a.nviews));
+  FC_CHECK_ARG((long long)a.nbz * a.nsplit <= 65535, "volume too large for the adjoint grid");
   const size_t nwin = (size_t)a.nbx * a.nby * a.nbz * a.nviews;
   void* ws = nullptr;
   // window words, then one 256-byte-aligned word for max |y|
@@ -1067,7 +1076,7 @@ int launch_adjoint(int device, float* vol, const int vs[3], const double origin[
   else if (dt == FC_BF16) absmax_kernel<FC_BF16><<<rblocks, 256, 0, stream>>>(proj, norm, (float)h, per_view, a.nviews, ymax);
   else absmax_kernel<FC_F16><<<rblocks, 256, 0, stream>>>(proj, norm, (float)h, per_view, a.nviews, ymax);
   adj_windows<<<(unsigned)((nwin + 255) / 256), 256, 0, stream>>>(a, static_cast<uint2*>(ws));
-  const dim3 grid(a.nbx, a.nby, a.nbz);
+  const dim3 grid(a.nbx, a.nby, a.nbz * a.nsplit);
   if (dt == FC_F32) adjoint_vox_kernel<FC_F32><<<grid, 256, 0, stream>>>(a);
   else if (dt == FC_BF16) adjoint_vox_kernel<FC_BF16><<<grid, 256, 0, stream>>>(a);
   else adjoint_vox_kernel<FC_F16><<<grid, 256, 0, stream>>>(a);

@@ -109,8 +109,10 @@ class UNet3D(torch.nn.Module):
         return conv3d(x, self.w[name], self.b[name], relu)
 
     def forward(self, vol):
-        """vol: [N, D, H, W] FP32 (FDK reconstruction) -> [N, D, H, W] FP32 prediction."""
-        if vol.dim() == 3:
+        """vol: [N, D, H, W] (or [D, H, W]) FP32 FDK reconstruction -> FP32 prediction of the
+        same shape."""
+        single = vol.dim() == 3
+        if single:
             vol = vol[None]
         for s in vol.shape[1:]:
             if s % 4:
@@ -121,8 +123,8 @@ class UNet3D(torch.nn.Module):
         bt = self._c(self._c(avgpool2(e2), "b_a"), "b_b")
         d2 = self._c(self._c(torch.cat([upsample2(bt), e2], dim=-1).contiguous(), "d2a"), "d2b")
         d1 = self._c(self._c(torch.cat([upsample2(d2), e1], dim=-1).contiguous(), "d1a"), "d1b")
-        out = self._c(d1, "out")[..., 0]
-        return vol + out / self.SCALE
+        out = vol + self._c(d1, "out")[..., 0] / self.SCALE
+        return out[0] if single else out
 
     def save(self, path):
         np.savez(path, **{f"w_{k}": v.detach().cpu().numpy() for k, v in self.w.items()},

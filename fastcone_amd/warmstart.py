@@ -118,7 +118,7 @@ def train(X, Y, epochs, batch=4, lr=1e-3, seed=0, log=print, val_frac=0.05, resu
             opt.zero_grad(set_to_none=True)
             loss.backward()
             opt.step()
-            tot += float(loss)
+            tot += float(loss.detach())
         with torch.no_grad():
             xv = X[n - nval:].float()
             yv = Y[n - nval:].float()
@@ -147,6 +147,10 @@ def evaluate(net, device, seed=777, tol=1e-3, log=print):
     vol = phantom.voxelize(ell, geom.vol_shape, 1.0, device)
     b = poisson_log(A(vol), I0, scale=1.0, seed=seed)
     atb = float(norm64(At(b)))
+    # untimed warm-up of every piece (FFT plans, workspaces, kernels)
+    with torch.no_grad():
+        net(fdk(b, geom))
+    torch.cuda.synchronize()
     # plain: FDK start
     torch.cuda.synchronize()
     t0 = time.perf_counter()
