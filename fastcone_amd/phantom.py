@@ -95,10 +95,11 @@ def config_phantom(config, device=None, seed=None):
         e = random_ellipsoids(1 if seed is None else seed, 40, (3, 80), (0.002, 0.03), 60.0)
         return voxelize(e, (512, 512, 512), 0.5, device, texture_sigma=0.001,
                         texture_seed=1 if seed is None else seed), e
-    if config == 2:
-        e = random_ellipsoids(2 if seed is None else seed, 10, (5, 40), (0.005, 0.02), 40.0,
+    if config == 2:  # as configuration 1, elongated along z
+        e = random_ellipsoids(2 if seed is None else seed, 40, (3, 80), (0.002, 0.03), 60.0,
                               z_axes_mm=(20, 120))
-        return voxelize(e, (256, 512, 512), 0.5, device), e
+        return voxelize(e, (256, 512, 512), 0.5, device, texture_sigma=0.001,
+                        texture_seed=2 if seed is None else seed), e
     if config == 3:
         e = random_ellipsoids(3 if seed is None else seed, 10, (5, 40), (0.005, 0.02), 40.0)
         return voxelize(e, (256, 256, 256), 1.0, device), e
