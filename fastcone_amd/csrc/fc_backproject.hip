@@ -189,7 +189,8 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
   for (int k = 0; k < kZR; ++k) acc[k] = 0.0f;
 
   // Rows of a view's window, staged as (row wave + 4 q, column lane); zero outside the detector.
-  float pre[kPre];
+  // The registers hold raw storage bits: converting at commit keeps the loads in flight.
+  uint32_t pre[kPre];
   auto issue = [&](int t, uint2 w) {
     const uint32_t Wu = w.y & 0xffffu, Wv = w.y >> 16;
     if (Wu == kWinSkip || Wu == kWinGlobal) return;
@@ -201,9 +202,9 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
     for (int q = 0; q < kPre; ++q) {
       const int r = wave + 4 * q;
       const int pv = v0 + r;
-      pre[q] = 0.0f;
+      pre[q] = 0u;  // +0 in every storage format
       if (colok && (uint32_t)r < Wv && pv >= 0 && pv < a.nh)
-        pre[q] = load_elem<DT>(a.proj, vbase + (size_t)pv * a.nw + pu);
+        pre[q] = load_bits<DT>(a.proj, vbase + (size_t)pv * a.nw + pu);
     }
   };
   auto commit = [&](float* T, uint2 w) {
@@ -212,7 +213,7 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
 #pragma unroll
     for (int q = 0; q < kPre; ++q) {
       const uint32_t r = (uint32_t)(wave + 4 * q);
-      if ((uint32_t)lane < Wu && r < Wv) T[r * Wu + (uint32_t)lane] = pre[q];
+      if ((uint32_t)lane < Wu && r < Wv) T[r * Wu + (uint32_t)lane] = bits_to_f32<DT>(pre[q]);
     }
   };
 

@@ -53,6 +53,27 @@ __device__ __forceinline__ float load_elem(const void* p, size_t i) {
   }
 }
 
+// Raw storage bits of element i (FP32 bits, or the 16-bit pattern in the low half) and their
+// conversion, split so that a prefetch can keep loads in flight and convert only when it stores.
+template <int DT>
+__device__ __forceinline__ uint32_t load_bits(const void* p, size_t i) {
+  if constexpr (DT == FC_F32) {
+    return static_cast<const uint32_t*>(p)[i];
+  } else {
+    return static_cast<const uint16_t*>(p)[i];
+  }
+}
+template <int DT>
+__device__ __forceinline__ float bits_to_f32(uint32_t b) {
+  if constexpr (DT == FC_F32) {
+    return __uint_as_float(b);
+  } else if constexpr (DT == FC_BF16) {
+    return __uint_as_float(b << 16);
+  } else {
+    return f16_bits_to_f32(b);
+  }
+}
+
 template <int DT>
 __device__ __forceinline__ uint16_t to_bits16(float f) {
   if constexpr (DT == FC_BF16) {

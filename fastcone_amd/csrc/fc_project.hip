@@ -425,17 +425,17 @@ __global__ __launch_bounds__(256) void fp_lds_kernel(FpArgs a) {
 #pragma unroll 1
         for (int q0 = 0; q0 < kLdsCap / 256; q0 += kChunk) {
           if ((int)threadIdx.x + 256 * q0 >= NE) break;
-          float v[kChunk];
+          uint32_t v[kChunk];  // raw storage bits: the conversion waits for nothing
 #pragma unroll
           for (int q = 0; q < kChunk; ++q) {
             const int e = threadIdx.x + 256 * (q0 + q);
-            v[q] = 0.0f;
-            if (e < NE) v[q] = load_elem<DT>(a.data, brick_elem_offset<DP>(P, B, e, invE1));
+            v[q] = 0u;
+            if (e < NE) v[q] = load_bits<DT>(a.data, brick_elem_offset<DP>(P, B, e, invE1));
           }
 #pragma unroll
           for (int q = 0; q < kChunk; ++q) {
             const int e = threadIdx.x + 256 * (q0 + q);
-            if (e < NE) brick[e] = v[q];
+            if (e < NE) brick[e] = bits_to_f32<DT>(v[q]);
           }
         }
       } else {
