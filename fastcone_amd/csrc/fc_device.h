@@ -115,6 +115,13 @@ __device__ __forceinline__ uint32_t mad_u24(uint32_t a, uint32_t b, uint32_t c) 
   asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
   return r;
 }
+// a * b for a, b < 2^24 (kept as v_mul_u32_u24: the compiler would otherwise fold a product
+// with a constant into a quarter-rate 32-bit multiply)
+__device__ __forceinline__ uint32_t mul_u24(uint32_t a, uint32_t b) {
+  uint32_t r;
+  asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
 
 // Logical block index for a 1-D grid, remapped so each of the 8 XCDs (which receive blocks
 // round-robin) works on one contiguous range of logical blocks.

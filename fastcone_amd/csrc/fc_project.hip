@@ -335,19 +335,22 @@ __device__ __forceinline__ SlabBox slab_box(const SlabPlan& P, int m) {
   return B;
 }
 
-// Global element offset of brick element e = d + DP (m1 + E1 m2) (d fastest). Planes past the
-// padded block (last slab) are clamped to its last plane; they are never sampled.
+// Global element offset of brick element e = d + DP (m1 + E1 m2) (d fastest), given the
+// slab's base offset gbase = dlo gsd + e1lo gs1 + e2lo gs2. Planes past the padded block (last
+// slab) are clamped to its last plane (dmax = PD - 1 - dlo); they are never sampled. The
+// quotients are exact float products: e < 2^13 and rest < 2^10, and adding 1/2 keeps each
+// quotient at least 1/(2 max(DP, E1)) away from an integer, far above the FP32 rounding. All
+// multiplies are 24-bit (full rate; the launcher keeps PX * PY < 2^24), where a 32-bit integer
+// multiply or divide would issue at a quarter of the rate.
 template <int DP>
 __device__ __forceinline__ uint32_t brick_elem_offset(const SlabPlan& P, const SlabBox& B,
-                                                      uint32_t e, float invE1) {
-  const uint32_t d = e % DP, rest = e / DP;
-  // exact floor(rest / E1) for rest < 2^16: rest + 1/2 keeps the quotient 1/(2 E1) away from
-  // an integer, far above the FP32 rounding of the product
+                                                      uint32_t e, float invE1, uint32_t gbase,
+                                                      uint32_t dmax) {
+  const uint32_t rest = (uint32_t)(((float)e + 0.5f) * (1.0f / (float)DP));
+  const uint32_t d = e - mul_u24(rest, (uint32_t)DP);
   const uint32_t m2 = (uint32_t)(((float)rest + 0.5f) * invE1);
-  const uint32_t m1 = rest - m2 * (uint32_t)B.E1;
-  const uint32_t dd = min(d, (uint32_t)(P.PD - 1 - B.dlo));
-  return (uint32_t)B.dlo * P.gsd + (uint32_t)B.e1lo * P.gs1 + (uint32_t)B.e2lo * P.gs2 +
-         dd * P.gsd + m1 * P.gs1 + m2 * P.gs2;
+  const uint32_t m1 = rest - mul_u24(m2, (uint32_t)B.E1);
+  return mad_u24(min(d, dmax), P.gsd, mad_u24(m1, P.gs1, mad_u24(m2, P.gs2, gbase)));
 }
 
 // Trilinear sample from an LDS brick laid out dominant axis fastest (pitch DP), then minor axis 1
@@ -419,6 +422,9 @@ __global__ __launch_bounds__(256) void fp_lds_kernel(FpArgs a) {
       // elements along the dominant axis unrolled (LDS index d + DP c, pitch DP odd so the
       // writes are bank-conflict free). Planes past the padded block (last slab) read its last
       // plane and are never sampled.
+      const uint32_t gbase = mad_u24((uint32_t)B.dlo, P.gsd,
+                                     mad_u24((uint32_t)B.e1lo, P.gs1, (uint32_t)B.e2lo * P.gs2));
+      const uint32_t dmax = (uint32_t)(P.PD - 1 - B.dlo);  // >= 2: dlo = b S <= PD - 3
       if (P.gsd == 1u) {
         const float invE1 = 1.0f / (float)B.E1;
         constexpr int kChunk = 8;
@@ -430,7 +436,8 @@ __global__ __launch_bounds__(256) void fp_lds_kernel(FpArgs a) {
           for (int q = 0; q < kChunk; ++q) {
             const int e = threadIdx.x + 256 * (q0 + q);
             v[q] = 0u;
-            if (e < NE) v[q] = load_bits<DT>(a.data, brick_elem_offset<DP>(P, B, e, invE1));
+            if (e < NE)
+              v[q] = load_bits<DT>(a.data, brick_elem_offset<DP>(P, B, e, invE1, gbase, dmax));
           }
 #pragma unroll
           for (int q = 0; q < kChunk; ++q) {
@@ -441,18 +448,15 @@ __global__ __launch_bounds__(256) void fp_lds_kernel(FpArgs a) {
       } else {
         const int ncol = B.E1 * B.E2;
         const float invE1 = 1.0f / (float)B.E1;
-        const uint32_t g00 = (uint32_t)B.dlo * P.gsd + (uint32_t)B.e1lo * P.gs1 +
-                             (uint32_t)B.e2lo * P.gs2;
-        const int dmax = P.PD - 1 - B.dlo;  // >= 2: dlo = b S <= PD - 3
 #pragma unroll 1
         for (int c = threadIdx.x; c < ncol; c += 256) {
-          const uint32_t m2 = (uint32_t)(((float)c + 0.5f) * invE1);  // exact: c < 2^16
-          const uint32_t m1 = (uint32_t)c - m2 * (uint32_t)B.E1;
-          const uint32_t g = g00 + m1 * P.gs1 + m2 * P.gs2;
+          const uint32_t m2 = (uint32_t)(((float)c + 0.5f) * invE1);  // exact: c < 2^10
+          const uint32_t m1 = (uint32_t)c - mul_u24(m2, (uint32_t)B.E1);
+          const uint32_t g = mad_u24(m1, P.gs1, mad_u24(m2, P.gs2, gbase));
           float v[DP];
 #pragma unroll
           for (int d = 0; d < DP; ++d)
-            v[d] = load_elem<DT>(a.data, g + (uint32_t)min(d, dmax) * P.gsd);
+            v[d] = load_elem<DT>(a.data, mad_u24(min((uint32_t)d, dmax), P.gsd, g));
 #pragma unroll
           for (int d = 0; d < DP; ++d) brick[d + DP * c] = v[d];
         }
@@ -929,6 +933,11 @@ int launch_project(int device, const void* vol, int dt, const int vs[3], const d
   const int nx = vs[0], ny = vs[1], nz = vs[2];
   int layout = fp_layout_choice();
   if (layout == FP_PLAIN && dt != FC_F32) layout = FP_LDS;
+  // the LDS projector's brick addressing is 24-bit: it needs (nx + 4)(ny + 4) < 2^24
+  const bool big_slice = (size_t)(nx + 4) * (size_t)(ny + 4) >= (1u << 24);
+  if (layout == FP_LDS && big_slice && dt == FC_F32) layout = FP_PLAIN;
+  FC_CHECK_ARG(!(layout == FP_LDS && big_slice),
+               "16-bit storage needs (nx + 4)(ny + 4) < 2^24 (tile the volume in x or y)");
   int PX, PY, PZ;
   size_t bytes;
   if (layout == FP_OCTET) {

@@ -221,26 +221,56 @@ def fbp(vol, proj, projAngles, specimenLength, zOrigin, volStart=np.array([0, 0,
     return reScale
 
 
-def ramp_response(nw):
+def ramp_response(nw, window=None, scale=0.125):
     """Ramp of fastcone.py:278-281 on the zero-padded length n = 2 Nw: H[k] = pi k / n for
-    k = 1..Nw and H[0] = pi * 0.125 / n (rfft bins)."""
+    k = 1..Nw and H[0] = pi * 0.125 / n (rfft bins). window "cos" multiplies by cos(pi f) and
+    "gauss" by exp(-f^2 / (2 scale^2)), f = k / n in cycles per pixel (the windows the reference's
+    cosrampfilter / gaussrampfilter aim at, fastcone.py:294-319)."""
     n = 2 * int(nw)
-    H = np.pi * np.arange(n // 2 + 1, dtype=np.float64) / n
+    f = np.arange(n // 2 + 1, dtype=np.float64) / n
+    H = np.pi * f
     H[0] = np.pi * 0.125 / n
+    if window == "cos":
+        H *= np.cos(np.pi * f)
+    elif window == "gauss":
+        H *= np.exp(-0.5 * (f / float(scale)) ** 2)
+    elif window is not None:
+        raise ValueError(f"unknown ramp window {window!r}")
     return H
 
 
-def rampfilter(proj):
-    """Ramp-filter every view along w in place (fastcone.py:272-292): zero-pad to 2 Nw with the
-    data left-aligned, multiply the FP64 rfft by the ramp, inverse transform, crop [0, Nw)."""
+def _filter_views(proj, H):
     nt, nh, nw = proj.shape
-    H = ramp_response(nw)
     chunk = max(1, (1 << 24) // max(1, nh * nw))  # bound the FP64 temporaries
     for t0 in range(0, nt, chunk):
         t1 = min(nt, t0 + chunk)
         spec = np.fft.rfft(proj[t0:t1].astype(np.float64), n=2 * nw, axis=2)
         spec *= H
         proj[t0:t1] = np.fft.irfft(spec, n=2 * nw, axis=2)[..., :nw]
+
+
+def rampfilter(proj):
+    """Ramp-filter every view along w in place (fastcone.py:272-292): zero-pad to 2 Nw with the
+    data left-aligned, multiply the FP64 rfft by the ramp, inverse transform, crop [0, Nw)."""
+    _filter_views(proj, ramp_response(proj.shape[2]))
+    return
+
+
+def cosrampfilter(proj):
+    """Cosine-windowed ramp filter, in place (fastcone.py:294-305, fixed). The reference does not
+    run as written: it transforms back only after its loop over views (so one view is filtered)
+    and scales its cosine by 1/Nw (so the window is ~1). Here every view gets rampfilter's padded
+    ramp times cos(pi f), which falls to zero at the Nyquist frequency."""
+    _filter_views(proj, ramp_response(proj.shape[2], "cos"))
+    return
+
+
+def gaussrampfilter(proj, scale=0.125):
+    """Gaussian-windowed ramp filter, in place (fastcone.py:307-319, fixed: the reference uses
+    scipy without importing it and transforms back only after its loop). The window is the
+    reference's Gaussian over frequency indices with standard deviation scale * Nw, i.e.
+    exp(-f^2 / (2 scale^2)) in cycles per pixel, on rampfilter's padded ramp."""
+    _filter_views(proj, ramp_response(proj.shape[2], "gauss", scale))
     return
 
 

@@ -177,3 +177,33 @@ def test_ramp_response_matches_oracle():
     for nw in (8, 33, 768):
         assert np.allclose(fastcone.ramp_response(nw), oracle.ramp_filter_response(nw),
                            rtol=1e-15, atol=0)
+
+
+def test_gaussrampfilter_window_matches_scipy_gaussian():
+    # the window is a Gaussian over frequency indices with standard deviation scale * Nw (the
+    # reference builds it with scipy's gaussian window centred on the DC bin)
+    from scipy.signal import windows
+    nw, scale = 64, 0.125
+    ratio = fastcone.ramp_response(nw, "gauss", scale) / fastcone.ramp_response(nw)
+    g = windows.gaussian(nw + 1, scale * nw)  # odd length: index nw // 2 is the centre
+    for k in range(nw // 2 + 1):
+        assert abs(ratio[2 * k] - g[nw // 2 + k]) < 1e-12  # padded bin 2k is frequency k / Nw
+
+
+def test_windowed_ramp_filters():
+    rng = np.random.default_rng(4)
+    p = rng.standard_normal((3, 4, 40)).astype(np.float32)
+    wide, plain = p.copy(), p.copy()
+    fastcone.gaussrampfilter(wide, scale=1e6)  # window -> 1
+    fastcone.rampfilter(plain)
+    assert np.allclose(wide, plain, rtol=1e-6, atol=1e-6)
+    H = fastcone.ramp_response(40, "cos")
+    assert abs(H[-1]) < 1e-15 and np.all(H <= fastcone.ramp_response(40) + 1e-15)
+    c = p.copy()
+    fastcone.cosrampfilter(c)
+    # every view is filtered, and the cosine window removes energy near the Nyquist frequency
+    hi = lambda a: np.abs(np.fft.rfft(a.astype(np.float64), axis=2)[..., -6:]).sum()
+    assert np.all([np.abs(c[t] - p[t]).max() > 0 for t in range(3)])
+    assert hi(c) < 0.5 * hi(plain)
+    with pytest.raises(ValueError):
+        fastcone.ramp_response(8, "hann")
