@@ -527,23 +527,26 @@ __device__ __forceinline__ void scatter_global(float* accp, uint32_t PX, uint32_
 // Voxel-owned exact transpose of the projector. A block owns the 16^3 "base cells" of a brick
 // (voxel indices [16 b - 1, 16 b + 15) per axis, which tile [-1, n - 1]): it walks every ray of
 // every view whose detector window (adj_windows) can reach the brick, recomputes the ray's samples
-// exactly as the projector does (same ray_segment, same FMA positions), keeps those whose base
+// exactly as the projector does (the ray's ray_segment, stored once by adj_rays; the same FMA
+// positions), keeps those whose base
 // corner floor(q) lies in the brick — so every sample of every ray is scattered by exactly one
 // block — and adds y h norm times its eight trilinear weights into a 17^3 LDS accumulator. The
-// accumulator is 64-bit fixed point: each contribution is scaled by 2^30 / max|y h norm| (a device
-// reduction over the projections, so |contribution| < 2^30), rounded to an integer and added with
+// accumulator is 64-bit fixed point: each contribution is scaled by 2^28 / max|y h norm| (a device
+// reduction over the projections, so |contribution| < 2^28), truncated to an integer and added with
 // ds_add_u64, which runs about 30x faster than an LDS float atomic on this GPU; the resolution is
-// 2^-30 of the largest datum and the 64-bit sums cannot overflow. The accumulator is converted
+// 2^-28 of the largest datum and the 64-bit sums cannot overflow. The accumulator is converted
 // back and added to the volume once at the end (one global float atomic per nonzero element;
 // neighbouring bricks share one face). (ray, view) items are dealt to the threads round-robin
 // across views, so lanes stay busy when windows are small.
 constexpr int kAB = 16;             // base cells per brick edge
 constexpr int kAL = kAB + 1;        // accumulator edge (corners reach one cell further)
 constexpr int kAL3 = kAL * kAL * kAL;
+constexpr int kAT = 512;           // threads per brick (8 waves share one accumulator)
 
 struct AdjVArgs {
   float* vol;          // unpadded output block, accumulated
-  const unsigned int* ymax;  // bits of max |y h norm| over the stack (non-negative float)
+  unsigned int* ymax;  // bits of max |y h norm| over the stack (non-negative float)
+  float4* rays;        // per ray: (px, py, pz, sx), (sy, sz, nk bits, y h norm); see adj_rays
   const void* proj;
   const double* views;
   const float* norm;
@@ -557,19 +560,32 @@ struct AdjVArgs {
   int max_steps;
 };
 
-// max |proj norm h| over the stack, as the bits of a non-negative float (ordered like the value):
+// One thread per ray (view, row, column): the ray's FP32 first sample, step and sample count
+// exactly as the projector computes them (ray_segment), and its datum y h norm, stored once so
+// that every brick the ray crosses reads 32 bytes instead of redoing the FP64 set-up; plus
+// max |y h norm| over the stack as the bits of a non-negative float (ordered like the value):
 // a block-wide maximum, then one integer atomic per block.
 template <int DT>
-__global__ __launch_bounds__(256) void absmax_kernel(const void* __restrict__ proj,
-                                                     const float* __restrict__ norm, float h,
-                                                     size_t per_view, int nviews,
-                                                     unsigned int* __restrict__ out) {
+__global__ __launch_bounds__(256) void adj_rays(AdjVArgs a) {
   __shared__ float red[256];
+  const size_t per_view = (size_t)a.nw * a.nh;
+  const size_t total = per_view * (size_t)a.nviews;
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
   float m = 0.0f;
-  const size_t total = per_view * (size_t)nviews;
-  for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (size_t)gridDim.x * 256) {
-    const float s = norm ? norm[e / per_view] : 1.0f;
-    m = fmaxf(m, fabsf(load_elem<DT>(proj, e) * s * h));
+  if (idx < total) {
+    const int t = (int)(idx / per_view);
+    const int rem = (int)(idx - (size_t)t * per_view);
+    const int i = rem / a.nw, j = rem - i * a.nw;
+    const float y = load_elem<DT>(a.proj, idx) * (a.norm ? a.norm[t] : 1.0f) * (float)a.h;
+    RaySeg r;
+    r.nk = 0;
+    r.px = r.py = r.pz = r.sx = r.sy = r.sz = 0.0f;
+    if (y != 0.0f)
+      r = ray_segment(a.views + (size_t)t * FC_VIEW_DOUBLES, j, i, a.ox, a.oy, a.oz, a.nx, a.ny,
+                      a.nz, a.h, a.max_steps, 2.0f);
+    a.rays[2 * idx] = make_float4(r.px, r.py, r.pz, r.sx);
+    a.rays[2 * idx + 1] = make_float4(r.sy, r.sz, __int_as_float(r.nk), r.nk > 0 ? y : 0.0f);
+    m = r.nk > 0 ? fabsf(y) : 0.0f;
   }
   red[threadIdx.x] = m;
   __syncthreads();
@@ -577,7 +593,7 @@ __global__ __launch_bounds__(256) void absmax_kernel(const void* __restrict__ pr
     if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) atomicMax(out, __float_as_uint(red[0]));
+  if (threadIdx.x == 0) atomicMax(a.ymax, __float_as_uint(red[0]));
 }
 
 // One thread per (brick, view): the detector pixels whose rays can carry samples with a base cell
@@ -622,13 +638,12 @@ __global__ __launch_bounds__(256) void adj_windows(AdjVArgs a, uint2* __restrict
   out[idx] = w;
 }
 
-template <int DT>
-__global__ __launch_bounds__(256) void adjoint_vox_kernel(AdjVArgs a) {
+__global__ __launch_bounds__(kAT) void adjoint_vox_kernel(AdjVArgs a) {
   __shared__ unsigned long long acc[kAL3];
-  for (int e = threadIdx.x; e < kAL3; e += 256) acc[e] = 0ull;
+  for (int e = threadIdx.x; e < kAL3; e += kAT) acc[e] = 0ull;
   const float ymax = __uint_as_float(*a.ymax);
   if (!(ymax > 0.0f) || !(ymax < 3.0e38f)) return;  // all-zero data (uniform exit)
-  const float fscale = 1073741824.0f / ymax;        // 2^30 / max|y|
+  const float fscale = 268435456.0f / ymax;         // 2^28 / max|y|
   __syncthreads();
   const int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z % a.nbz;
   // this block's share of the views (small volumes split the views over several blocks)
@@ -641,7 +656,11 @@ __global__ __launch_bounds__(256) void adjoint_vox_kernel(AdjVArgs a) {
   const uint2* W = a.win + ((size_t)(bz * a.nby + by) * a.nbx + bx) * (size_t)a.nviews;
   int t = tbeg, start = 0;
   uint2 w = tbeg < tend ? W[tbeg] : make_uint2(0u, 0u);
-  for (int g = threadIdx.x;; g += 256) {
+  // Items are dealt in chunks of kAT: thread tid takes item (lane 8 + wave) of each chunk, so the
+  // 64 lanes of a wave hold rays 8 pixels apart and rarely add to the same LDS word at once
+  // (same-address atomics serialise).
+  const int perm = ((threadIdx.x & 63) << 3) | (threadIdx.x >> 6);
+  for (int g = perm;; g += kAT) {
     int Wj = (int)(w.y & 0xffffu), npix = Wj * (int)(w.y >> 16);
     while (t < tend && g >= start + npix) {
       start += npix;
@@ -654,13 +673,14 @@ __global__ __launch_bounds__(256) void adjoint_vox_kernel(AdjVArgs a) {
     if (t >= tend) break;
     const int p = g - start;
     const int i = (int)(w.x >> 16) + p / Wj, j = (int)(w.x & 0xffffu) + p % Wj;
-    const float y = load_elem<DT>(a.proj, ((size_t)t * a.nh + i) * a.nw + j) *
-                    (a.norm ? a.norm[t] : 1.0f) * (float)a.h * fscale;  // |y| <= 2^30
-    if (y == 0.0f) continue;
-    const double* V = a.views + (size_t)t * FC_VIEW_DOUBLES;
-    const RaySeg r = ray_segment(V, j, i, a.ox, a.oy, a.oz, a.nx, a.ny, a.nz, a.h, a.max_steps,
-                                 2.0f);
-    if (r.nk == 0) continue;
+    const size_t ray = ((size_t)t * a.nh + i) * a.nw + j;
+    const float4 r1 = a.rays[2 * ray + 1];
+    if (r1.w == 0.0f) continue;  // zero datum or a ray that misses the block
+    const float4 r0 = a.rays[2 * ray];
+    RaySeg r;
+    r.px = r0.x; r.py = r0.y; r.pz = r0.z; r.sx = r0.w;
+    r.sy = r1.x; r.sz = r1.y; r.nk = __float_as_int(r1.z);
+    const float y = r1.w * fscale;  // |y| <= 2^28
     // k range of samples whose base cell can lie in the brick: per axis, q in [lo, lo + 16);
     // widened by one step for FP32 rounding, then every sample is checked exactly
     float klo = 0.0f, khi = (float)(r.nk - 1);
@@ -678,6 +698,25 @@ __global__ __launch_bounds__(256) void adjoint_vox_kernel(AdjVArgs a) {
     clip(r.pz, r.sz, flz);
     if (!(khi >= klo)) continue;
     const int k0 = max(0, (int)ceilf(klo)), k1 = min(r.nk - 1, (int)floorf(khi));
+    // Consecutive samples (0.5 voxel apart) often share a base cell: their weights are summed in
+    // registers and added to LDS when the cell changes or after kRun samples (so that a sum
+    // stays below 2^31 in the 2^28 fixed-point scale).
+    constexpr int kRun = 4;
+    float A[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) A[c] = 0.0f;
+    uint32_t cur = 0xffffffffu;
+    int run = 0;
+    auto flush = [&]() {
+      unsigned long long* L = acc + cur;
+      // truncate to an integer, sign-extend, add in 64-bit two's complement
+      auto add = [](unsigned long long* p, float v) {
+        atomicAdd(p, (unsigned long long)(long long)(int)v);
+      };
+      add(L, A[0]); add(L + 1, A[1]); add(L + kAL, A[2]); add(L + kAL + 1, A[3]);
+      add(L + kAL * kAL, A[4]); add(L + kAL * kAL + 1, A[5]);
+      add(L + kAL * kAL + kAL, A[6]); add(L + kAL * kAL + kAL + 1, A[7]);
+    };
     for (int k = k0; k <= k1; ++k) {
       const float kf = (float)k;
       const float qx = fmaf(kf, r.sx, r.px), qy = fmaf(kf, r.sy, r.py), qz = fmaf(kf, r.sz, r.pz);
@@ -686,25 +725,29 @@ __global__ __launch_bounds__(256) void adjoint_vox_kernel(AdjVArgs a) {
       if (cx >= (uint32_t)kAB || cy >= (uint32_t)kAB || cz >= (uint32_t)kAB) continue;
       const float ax = __builtin_amdgcn_fractf(qx), ay = __builtin_amdgcn_fractf(qy),
                   az = __builtin_amdgcn_fractf(qz);
-      unsigned long long* L = acc + (cz * kAL + cy) * kAL + cx;
+      const uint32_t cell = (cz * kAL + cy) * kAL + cx;
       const float wz1 = y * az, wz0 = y - wz1;
       const float w10 = wz0 * ay, w00 = wz0 - w10, w11 = wz1 * ay, w01 = wz1 - w11;
-      // round to the nearest integer, sign-extend, add in 64-bit two's complement
-      auto add = [](unsigned long long* p, float v) {
-        atomicAdd(p, (unsigned long long)(long long)(int)rintf(v));
-      };
-      float v;
-      v = w00 * ax; add(L + 1, v); add(L, w00 - v);
-      v = w10 * ax; add(L + kAL + 1, v); add(L + kAL, w10 - v);
-      v = w01 * ax; add(L + kAL * kAL + 1, v); add(L + kAL * kAL, w01 - v);
-      v = w11 * ax; add(L + kAL * kAL + kAL + 1, v); add(L + kAL * kAL + kAL, w11 - v);
+      const float w[8] = {w00 - w00 * ax, w00 * ax, w10 - w10 * ax, w10 * ax,
+                          w01 - w01 * ax, w01 * ax, w11 - w11 * ax, w11 * ax};
+      if (cell != cur || run == kRun) {
+        if (cur != 0xffffffffu) flush();
+        cur = cell;
+        run = 0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) A[c] = 0.0f;
+      }
+      ++run;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) A[c] += w[c];
     }
+    if (cur != 0xffffffffu) flush();
   }
   __syncthreads();
   // LDS element (ex, ey, ez) is voxel (16 b - 1 + e) of the block; voxels outside [0, n) are the
   // projector's zero halo
-  const double inv = (double)ymax / 1073741824.0;
-  for (int e = threadIdx.x; e < kAL3; e += 256) {
+  const double inv = (double)ymax / 268435456.0;
+  for (int e = threadIdx.x; e < kAL3; e += kAT) {
     const long long q = (long long)acc[e];
     if (q == 0) continue;
     const float v = (float)((double)q * inv);
@@ -1069,26 +1112,26 @@ int launch_adjoint(int device, float* vol, const int vs[3], const double origin[
   a.nsplit = (int)std::max(1LL, std::min<long long>((2048 + nbricks - 1) / nbricks, a.nviews));
   FC_CHECK_ARG((long long)a.nbz * a.nsplit <= 65535, "volume too large for the adjoint grid");
   const size_t nwin = (size_t)a.nbx * a.nby * a.nbz * a.nviews;
+  const size_t nrays = (size_t)a.nw * a.nh * a.nviews;
   void* ws = nullptr;
-  // window words, then one 256-byte-aligned word for max |y|
-  const size_t woff = (nwin * sizeof(uint2) + 255) / 256 * 256;
-  int rc = acquire_ws(device, SLOT_OCTET, woff + 256, stream, &ws);
+  // ray records (32 B each), window words, then one 256-byte-aligned word for max |y|
+  const size_t woff = nrays * 2 * sizeof(float4);
+  const size_t moff = woff + (nwin * sizeof(uint2) + 255) / 256 * 256;
+  int rc = acquire_ws(device, SLOT_OCTET, moff + 256, stream, &ws);
   if (rc) return rc;
-  a.win = static_cast<uint2*>(ws);
-  unsigned int* ymax = reinterpret_cast<unsigned int*>(static_cast<char*>(ws) + woff);
-  a.ymax = ymax;
+  a.rays = static_cast<float4*>(ws);
+  uint2* win = reinterpret_cast<uint2*>(static_cast<char*>(ws) + woff);
+  a.win = win;
+  a.ymax = reinterpret_cast<unsigned int*>(static_cast<char*>(ws) + moff);
   int tok = prof_begin("adjoint", stream);
-  zero_fill<unsigned int><<<1, 256, 0, stream>>>(ymax, 1);
-  const size_t per_view = (size_t)a.nw * a.nh;
-  const unsigned rblocks = (unsigned)std::min<size_t>((per_view * a.nviews + 255) / 256, 4096);
-  if (dt == FC_F32) absmax_kernel<FC_F32><<<rblocks, 256, 0, stream>>>(proj, norm, (float)h, per_view, a.nviews, ymax);
-  else if (dt == FC_BF16) absmax_kernel<FC_BF16><<<rblocks, 256, 0, stream>>>(proj, norm, (float)h, per_view, a.nviews, ymax);
-  else absmax_kernel<FC_F16><<<rblocks, 256, 0, stream>>>(proj, norm, (float)h, per_view, a.nviews, ymax);
-  adj_windows<<<(unsigned)((nwin + 255) / 256), 256, 0, stream>>>(a, static_cast<uint2*>(ws));
+  zero_fill<unsigned int><<<1, 256, 0, stream>>>(a.ymax, 1);
+  const unsigned rblocks = (unsigned)((nrays + 255) / 256);
+  if (dt == FC_F32) adj_rays<FC_F32><<<rblocks, 256, 0, stream>>>(a);
+  else if (dt == FC_BF16) adj_rays<FC_BF16><<<rblocks, 256, 0, stream>>>(a);
+  else adj_rays<FC_F16><<<rblocks, 256, 0, stream>>>(a);
+  adj_windows<<<(unsigned)((nwin + 255) / 256), 256, 0, stream>>>(a, win);
   const dim3 grid(a.nbx, a.nby, a.nbz * a.nsplit);
-  if (dt == FC_F32) adjoint_vox_kernel<FC_F32><<<grid, 256, 0, stream>>>(a);
-  else if (dt == FC_BF16) adjoint_vox_kernel<FC_BF16><<<grid, 256, 0, stream>>>(a);
-  else adjoint_vox_kernel<FC_F16><<<grid, 256, 0, stream>>>(a);
+  adjoint_vox_kernel<<<grid, kAT, 0, stream>>>(a);
   prof_end(tok, stream);
   FC_HIP(hipGetLastError());
   return release_ws(device, SLOT_OCTET, stream);
