@@ -189,7 +189,10 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
   for (int k = 0; k < kZR; ++k) acc[k] = 0.0f;
 
   // Rows of a view's window, staged as (row wave + 4 q, column lane); zero outside the detector.
-  // The registers hold raw storage bits: converting at commit keeps the loads in flight.
+  // The registers hold raw storage bits: converting at commit keeps the loads in flight. The
+  // row phase is wave-uniform, so row checks are scalar branches and addresses advance by a
+  // scalar stride.
+  const int wv = __builtin_amdgcn_readfirstlane(wave);
   uint32_t pre[kPre];
   auto issue = [&](int t, uint2 w) {
     const uint32_t Wu = w.y & 0xffffu, Wv = w.y >> 16;
@@ -197,23 +200,24 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
     const int u0 = (int)(int16_t)(w.x & 0xffffu), v0 = (int)(int16_t)(w.x >> 16);
     const int pu = u0 + lane;
     const bool colok = (uint32_t)lane < Wu && pu >= 0 && pu < a.nw;
-    const size_t vbase = (size_t)t * a.nh * a.nw;
+    const int pv0 = v0 + wv;
+    const long long row0 = ((long long)t * a.nh + pv0) * a.nw;  // element index of (pv0, 0)
 #pragma unroll
     for (int q = 0; q < kPre; ++q) {
-      const int r = wave + 4 * q;
-      const int pv = v0 + r;
       pre[q] = 0u;  // +0 in every storage format
-      if (colok && (uint32_t)r < Wv && pv >= 0 && pv < a.nh)
-        pre[q] = load_bits<DT>(a.proj, vbase + (size_t)pv * a.nw + pu);
+      const int pv = pv0 + 4 * q;
+      if ((uint32_t)(wv + 4 * q) < Wv && pv >= 0 && pv < a.nh && colok)
+        pre[q] = load_bits<DT>(a.proj, (size_t)(row0 + (long long)(4 * q) * a.nw + pu));
     }
   };
   auto commit = [&](float* T, uint2 w) {
     const uint32_t Wu = w.y & 0xffffu, Wv = w.y >> 16;
     if (Wu == kWinSkip || Wu == kWinGlobal) return;
+    float* Tw = T + (uint32_t)wv * Wu + (uint32_t)lane;
 #pragma unroll
     for (int q = 0; q < kPre; ++q) {
-      const uint32_t r = (uint32_t)(wave + 4 * q);
-      if ((uint32_t)lane < Wu && r < Wv) T[r * Wu + (uint32_t)lane] = bits_to_f32<DT>(pre[q]);
+      if ((uint32_t)(wv + 4 * q) < Wv && (uint32_t)lane < Wu)
+        Tw[4u * Wu * (uint32_t)q] = bits_to_f32<DT>(pre[q]);
     }
   };
 
