@@ -280,6 +280,12 @@ int fc_release_workspace(int device) {
   return release_all_ws(device);
 }
 
+// Pointer checks for the device entry points: an array may be null only when it is empty (a zero
+// extent); then the call has nothing to do once the device has been validated.
+static bool is_empty(const int shape[3]) {
+  return (long long)shape[0] * shape[1] * shape[2] == 0;
+}
+
 int fc_project_dev(int device, const void* vol, int vol_dtype, const int vol_shape[3],
                    const double origin[3], float* proj, const int proj_shape[3],
                    const double* views, double step, const float* norm, void* stream) {
@@ -287,12 +293,15 @@ int fc_project_dev(int device, const void* vol, int vol_dtype, const int vol_sha
   int rc = check_shapes(vol_shape, proj_shape);
   if (rc) return rc;
   FC_CHECK_ARG(dtype_ok(vol_dtype), "unknown volume dtype %d", vol_dtype);
-  FC_CHECK_ARG(vol && proj && views && origin, "null pointer argument");
+  FC_CHECK_ARG(origin && (vol || is_empty(vol_shape)) && (proj || is_empty(proj_shape)) &&
+                   (views || proj_shape[2] == 0),
+               "null pointer argument");
   FC_CHECK_ARG(std::isfinite(step) && step >= 1e-3 && step <= 1e3, "step %g out of [1e-3, 1e3]",
                step);
   for (int a = 0; a < 3; ++a) FC_CHECK_ARG(std::isfinite(origin[a]), "origin not finite");
   rc = select_device(device);
   if (rc) return rc;
+  if (is_empty(vol_shape) || is_empty(proj_shape)) return FC_OK;
   return launch_project(device, vol, vol_dtype, vol_shape, origin, proj, proj_shape, views, step,
                         norm, as_stream(stream));
 }
@@ -304,11 +313,14 @@ int fc_adjoint_dev(int device, float* vol, const int vol_shape[3], const double 
   int rc = check_shapes(vol_shape, proj_shape);
   if (rc) return rc;
   FC_CHECK_ARG(dtype_ok(proj_dtype), "unknown projection dtype %d", proj_dtype);
-  FC_CHECK_ARG(vol && proj && views && origin, "null pointer argument");
+  FC_CHECK_ARG(origin && (vol || is_empty(vol_shape)) && (proj || is_empty(proj_shape)) &&
+                   (views || proj_shape[2] == 0),
+               "null pointer argument");
   FC_CHECK_ARG(std::isfinite(step) && step >= 1e-3 && step <= 1e3, "step %g out of [1e-3, 1e3]",
                step);
   rc = select_device(device);
   if (rc) return rc;
+  if (is_empty(vol_shape) || is_empty(proj_shape)) return FC_OK;
   return launch_adjoint(device, vol, vol_shape, origin, proj, proj_dtype, proj_shape, views, step,
                         norm, as_stream(stream));
 }
@@ -321,9 +333,12 @@ int fc_backproject_dev(int device, float* vol, const int vol_shape[3], const dou
   if (rc) return rc;
   FC_CHECK_ARG(dtype_ok(proj_dtype), "unknown projection dtype %d", proj_dtype);
   FC_CHECK_ARG(mode >= 0 && mode <= 2, "unknown backprojection mode %d", mode);
-  FC_CHECK_ARG(vol && proj && views && origin, "null pointer argument");
+  FC_CHECK_ARG(origin && (vol || is_empty(vol_shape)) && (proj || is_empty(proj_shape)) &&
+                   (views || proj_shape[2] == 0),
+               "null pointer argument");
   rc = select_device(device);
   if (rc) return rc;
+  if (is_empty(vol_shape) || is_empty(proj_shape)) return FC_OK;
   return launch_backproject(device, vol, vol_shape, origin, proj, proj_dtype, proj_shape, views,
                             norm, mode, as_stream(stream));
 }
