@@ -335,24 +335,6 @@ __device__ __forceinline__ SlabBox slab_box(const SlabPlan& P, int m) {
   return B;
 }
 
-// Global element offset of brick element e = d + DP (m1 + E1 m2) (d fastest), given the
-// slab's base offset gbase = dlo gsd + e1lo gs1 + e2lo gs2. Planes past the padded block (last
-// slab) are clamped to its last plane (dmax = PD - 1 - dlo); they are never sampled. The
-// quotients are exact float products: e < 2^13 and rest < 2^10, and adding 1/2 keeps each
-// quotient at least 1/(2 max(DP, E1)) away from an integer, far above the FP32 rounding. All
-// multiplies are 24-bit (full rate; the launcher keeps PX * PY < 2^24), where a 32-bit integer
-// multiply or divide would issue at a quarter of the rate.
-template <int DP>
-__device__ __forceinline__ uint32_t brick_elem_offset(const SlabPlan& P, const SlabBox& B,
-                                                      uint32_t e, float invE1, uint32_t gbase,
-                                                      uint32_t dmax) {
-  const uint32_t rest = (uint32_t)(((float)e + 0.5f) * (1.0f / (float)DP));
-  const uint32_t d = e - mul_u24(rest, (uint32_t)DP);
-  const uint32_t m2 = (uint32_t)(((float)rest + 0.5f) * invE1);
-  const uint32_t m1 = rest - mul_u24(m2, (uint32_t)B.E1);
-  return mad_u24(min(d, dmax), P.gsd, mad_u24(m1, P.gs1, mad_u24(m2, P.gs2, gbase)));
-}
-
 // Trilinear sample from an LDS brick laid out dominant axis fastest (pitch DP), then minor axis 1
 // (E1 rows), then minor axis 2 (plane of zbytes bytes). `base` is the brick address minus the
 // brick origin's offset, so the corner of a sample at brick-frame-relative coordinates
@@ -416,9 +398,11 @@ __global__ __launch_bounds__(256) void fp_lds_kernel(FpArgs a) {
         continue;
       }
       __syncthreads();  // the previous brick is no longer read
-      // Fill. Dominant axis x (contiguous in HBM and in the brick): element order, every load
-      // of a chunk issued before its LDS writes. Dominant axis y or z: one thread per (minor1,
-      // minor2) column, minor axis 1 being x so a wave's loads stay coalesced, and the DP
+      // Fill. Dominant axis x (contiguous in HBM and in the brick): each lane keeps one offset d
+      // along x and walks rows, so 11 consecutive lanes read one row's consecutive x and an
+      // element costs one row decomposition; four rows' loads are issued before their LDS
+      // writes. Dominant axis y or z: one thread per (minor1, minor2) column, minor axis 1
+      // being x so a wave's loads stay coalesced, and the DP
       // elements along the dominant axis unrolled (LDS index d + DP c, pitch DP odd so the
       // writes are bank-conflict free). Planes past the padded block (last slab) read its last
       // plane and are never sampled.
@@ -426,23 +410,33 @@ __global__ __launch_bounds__(256) void fp_lds_kernel(FpArgs a) {
                                      mad_u24((uint32_t)B.e1lo, P.gs1, (uint32_t)B.e2lo * P.gs2));
       const uint32_t dmax = (uint32_t)(P.PD - 1 - B.dlo);  // >= 2: dlo = b S <= PD - 3
       if (P.gsd == 1u) {
+        // lanes 0..54 of each wave: d = lane % DP (dominant offset) and row phase lane / DP; the
+        // block's 4 waves cover 20 consecutive rows per step
         const float invE1 = 1.0f / (float)B.E1;
-        constexpr int kChunk = 8;
+        constexpr int kLanes = (64 / DP) * DP, kRowsW = 64 / DP, kRowsB = 4 * kRowsW;
+        if (lane < kLanes) {
+          const uint32_t rr = (uint32_t)(((float)lane + 0.5f) * (1.0f / (float)DP));
+          const uint32_t d = (uint32_t)lane - mul_u24(rr, (uint32_t)DP);
+          const uint32_t gd = gbase + min(d, dmax);
+          const int nrow = B.E1 * B.E2;
 #pragma unroll 1
-        for (int q0 = 0; q0 < kLdsCap / 256; q0 += kChunk) {
-          if ((int)threadIdx.x + 256 * q0 >= NE) break;
-          uint32_t v[kChunk];  // raw storage bits: the conversion waits for nothing
+          for (int r0 = kRowsW * wave + (int)rr; r0 < nrow; r0 += 4 * kRowsB) {
+            uint32_t v[4];
 #pragma unroll
-          for (int q = 0; q < kChunk; ++q) {
-            const int e = threadIdx.x + 256 * (q0 + q);
-            v[q] = 0u;
-            if (e < NE)
-              v[q] = load_bits<DT>(a.data, brick_elem_offset<DP>(P, B, e, invE1, gbase, dmax));
-          }
+            for (int q = 0; q < 4; ++q) {
+              const int r = r0 + q * kRowsB;
+              v[q] = 0u;
+              if (r < nrow) {
+                const uint32_t m2 = (uint32_t)(((float)r + 0.5f) * invE1);
+                const uint32_t m1 = (uint32_t)r - mul_u24(m2, (uint32_t)B.E1);
+                v[q] = load_bits<DT>(a.data, mad_u24(m1, P.gs1, mad_u24(m2, P.gs2, gd)));
+              }
+            }
 #pragma unroll
-          for (int q = 0; q < kChunk; ++q) {
-            const int e = threadIdx.x + 256 * (q0 + q);
-            if (e < NE) brick[e] = bits_to_f32<DT>(v[q]);
+            for (int q = 0; q < 4; ++q) {
+              const int r = r0 + q * kRowsB;
+              if (r < nrow) brick[d + DP * (uint32_t)r] = bits_to_f32<DT>(v[q]);
+            }
           }
         }
       } else {
