@@ -28,7 +28,7 @@ namespace fc {
 namespace {
 
 constexpr int kTile = 16;     // 16x16 pixel tile per 256-thread block, 8x8 per wave
-constexpr int kLdsCap = 6144;  // projector LDS brick budget, floats per block (24 KiB)
+constexpr int kLdsCap = 6144;  // fp_lds_kernel's LDS brick budget, floats per block (24 KiB)
 
 typedef float f32x2_a4 __attribute__((ext_vector_type(2), aligned(4)));
 
@@ -205,134 +205,137 @@ __global__ __launch_bounds__(256) void fp_kernel(FpArgs a) {
 
 // ---- LDS-staged projector ------------------------------------------------------------------------
 //
-// A block owns a 16x16 tile of rays of one view. The tile's rays are marched slab by slab along
-// the volume axis most aligned with the tile's central ray (the "dominant" axis). For each slab of
-// S voxel planes the block stages, in LDS, the brick that contains every trilinear corner the
-// tile's samples in that slab can touch, then each thread evaluates its ray's samples from LDS:
-//   * the brick's extent in the two minor axes is the bounding box of the tile's four corner rays
-//     between the slab planes (the tile's frustum slice), widened by one voxel each side;
+// Rays are marched slab by slab along the volume axis most aligned with a tile's rays (the
+// "dominant" axis). For each slab of S voxel planes the block stages, in LDS, the brick that
+// contains every trilinear corner its rays' samples in that slab can touch, then each thread
+// evaluates its ray's samples from LDS:
+//   * the brick's extent in the two minor axes is the bounding box of the corner rays of the
+//     block's tile(s) between the slab planes (the frustum slice), widened by one voxel each side;
 //   * along the dominant axis it spans S + 3 planes, [lo - 1, lo + S + 1];
 //   * LDS layout: dominant axis fastest with compile-time pitch S + 3, so the 8 corners of a
 //     sample are two base addresses with immediate offsets {0, 1, S+3, S+4};
 //   * a ray's samples are split between slabs by integer k ranges computed from the shared slab
 //     planes, so every sample is evaluated exactly once.
-// Bricks larger than the LDS budget (or tiles whose rays do not share a dominant direction) fall
-// back to gathering from the staged volume in global memory, with the same arithmetic.
+// Work is dealt in "quads": one 16x16 pixel tile in four consecutive views, four blocks. Adjacent
+// views' rays cross nearly the same voxels, so in "shared" mode each block takes one 8x8 quarter
+// of the tile in all four views (one view per wave) and its bricks are the union of the four
+// views' frustum slices: one fill serves four views (fp_quad_kernel, 8 waves per SIMD, 18 KiB
+// bricks). When the views are far apart (sparse angular sampling) that union is wide, and the
+// quad is left to fp_lds_kernel, where each block takes the whole 16x16 tile of one view
+// (6 waves per SIMD, 24 KiB bricks). fp_quad_kernel decides per quad from the brick elements of
+// both layouts summed over the slabs (a brick over its budget counting 4x) and records the
+// decision in a byte per quad that fp_lds_kernel, launched next on the same stream, reads.
+// Bricks larger than the LDS budget (or tiles whose rays do not all advance along the dominant
+// axis) fall back to gathering from the staged volume in global memory, with the same arithmetic.
 
-// Per-block slab plan shared by the LDS projector and the LDS adjoint: the dominant axis of the
-// tile's central ray, the permuted first sample / step of this thread's ray, and the tile's four
-// corner rays (for the frustum-slice boxes). Every field except the ray's is identical on all
-// threads of the block (same inputs, same instructions).
-struct SlabPlan {
+constexpr int kQuad = 4;          // views per quad
+constexpr int kQuadCap = 4608;    // fp_quad_kernel's LDS brick budget, floats (18 KiB)
+constexpr int kSlabTab = 64;      // slab boxes computed per table refill
+constexpr uint8_t kQuadShared = 1;
+
+// Frustum slice boxes. Each view's 16x16 tile is described by the rays through a 4x4 grid of
+// pixels, columns j0 + {0, 7, 8, 15} and rows i0 + {0, 7, 8, 15} (clamped to the detector): their
+// minor / dominant direction ratios (r1, r2), plus the view's source in the padded frame and its
+// interpolation margin. A sub-tile's four corner rays are the grid points (cj, ci) with
+// cj in {cj0, cj0 + 1 + cw}, ci in {ci0, ci0 + 1 + cw}: cw = 0 for an 8x8 quarter (cj0, ci0 in
+// {0, 2}), cw = 2 for the whole tile (cj0 = ci0 = 0).
+struct QuadGrid {
+  float r1[kQuad][16], r2[kQuad][16];
+  float src[kQuad][4];  // dominant, minor 1, minor 2, margin
+};
+
+// The block's dominant axis (of the tile-centre ray of view V), the two minor axes (minor axis 1
+// is x whenever x is not dominant, so brick columns load coalesced), the padded extents and
+// strides along them, and the slab count.
+struct SlabAxes {
   int ax, a1, a2;
   float dir;
-  float pd0, p10, p20, sd, s1, s2, inv_sd;
-  int nk;
-  int PD, P1, P2;
+  int PD, P1, P2, nslabs;
   uint32_t gsd, gs1, gs2;
-  float src_d, src_1, src_2;
-  float rc1[4], rc2[4];
-  float margin;
-  int nslabs;
-  bool corners_ok;  // all four corner rays advance along the dominant axis
 };
 
 template <int S>
-__device__ __forceinline__ void make_plan(SlabPlan& P, const double* V, const RaySeg& r, int j0,
-                                          int i0, int nw, int nh, double ox, double oy, double oz,
-                                          int PX, int PY, int nz) {
-  const double jc = fmin((double)j0 + 7.5, (double)(nw - 1));
-  const double ic = fmin((double)i0 + 7.5, (double)(nh - 1));
+__device__ __forceinline__ SlabAxes slab_axes(const FpArgs& a, const double* V, int j0, int i0) {
+  SlabAxes X;
   double cdx, cdy, cdz;
-  ray_dir(V, jc, ic, &cdx, &cdy, &cdz);
-  int ax = 0;
+  ray_dir(V, fmin((double)j0 + 7.5, (double)(a.nw - 1)), fmin((double)i0 + 7.5, (double)(a.nh - 1)),
+          &cdx, &cdy, &cdz);
+  X.ax = 0;
   double best = fabs(cdx);
-  if (fabs(cdy) > best) { ax = 1; best = fabs(cdy); }
-  if (fabs(cdz) > best) { ax = 2; }
-  const double cdom = ax == 0 ? cdx : (ax == 1 ? cdy : cdz);
-  P.ax = ax;
-  P.dir = cdom > 0.0 ? 1.0f : -1.0f;
-  // minor axis 1 is x whenever x is not dominant, so brick columns load coalesced
-  P.a1 = ax == 0 ? 1 : 0;
-  P.a2 = ax == 2 ? 1 : 2;
-  auto pick = [](int k, float x, float y, float z) { return k == 0 ? x : (k == 1 ? y : z); };
-  P.pd0 = pick(ax, r.px, r.py, r.pz);
-  P.p10 = pick(P.a1, r.px, r.py, r.pz);
-  P.p20 = pick(P.a2, r.px, r.py, r.pz);
-  P.sd = pick(ax, r.sx, r.sy, r.sz);
-  P.s1 = pick(P.a1, r.sx, r.sy, r.sz);
-  P.s2 = pick(P.a2, r.sx, r.sy, r.sz);
-  P.inv_sd = 1.0f / P.sd;
-  P.nk = r.nk;
-  const int PZ = nz + 4;
-  const int dimv[3] = {PX, PY, PZ};
-  const uint32_t strv[3] = {1u, (uint32_t)PX, (uint32_t)PX * (uint32_t)PY};
-  P.PD = dimv[ax]; P.P1 = dimv[P.a1]; P.P2 = dimv[P.a2];
-  P.gsd = strv[ax]; P.gs1 = strv[P.a1]; P.gs2 = strv[P.a2];
-  const double srcv[3] = {V[0] - ox + 2.0, V[1] - oy + 2.0, V[2] - oz + 2.0};
-  P.src_d = (float)srcv[ax];
-  P.src_1 = (float)srcv[P.a1];
-  P.src_2 = (float)srcv[P.a2];
-  const double jA = (double)j0, jB = (double)min(j0 + kTile - 1, nw - 1);
-  const double iA = (double)i0, iB = (double)min(i0 + kTile - 1, nh - 1);
-  // The slice of the tile's frustum between two slab planes is the hull of the corner rays'
-  // intersections only if every ray of the tile crosses the planes forwards. A flat tile's
-  // directions are affine in (j, i), so it suffices that each corner ray advances along the
-  // dominant axis (a corner ray that misses the volume is not checked by its thread's own test);
-  // 0.2 matches that per-ray test.
-  double umin = 1.0;
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    double dx, dy, dz;
-    ray_dir(V, (c & 1) ? jB : jA, (c & 2) ? iB : iA, &dx, &dy, &dz);
-    const double dv[3] = {dx, dy, dz};
-    const double rd = 1.0 / dv[ax];
-    P.rc1[c] = (float)(dv[P.a1] * rd);
-    P.rc2[c] = (float)(dv[P.a2] * rd);
-    umin = fmin(umin, dv[ax] * (double)P.dir / sqrt(dx * dx + dy * dy + dz * dz));
-  }
-  P.corners_ok = umin > 0.2;
-  P.margin = V[24] == (double)FC_DET_CYLINDER ? 2.0f : 1.0f;
-  P.nslabs = (P.PD - 4 + 1) / S + 1;  // slabs [1 + b S, 1 + (b+1) S) cover [1, n + 2]
+  if (fabs(cdy) > best) { X.ax = 1; best = fabs(cdy); }
+  if (fabs(cdz) > best) X.ax = 2;
+  X.dir = (X.ax == 0 ? cdx : (X.ax == 1 ? cdy : cdz)) > 0.0 ? 1.0f : -1.0f;
+  X.a1 = X.ax == 0 ? 1 : 0;
+  X.a2 = X.ax == 2 ? 1 : 2;
+  const int PZ = a.nz + 4;
+  X.PD = X.ax == 0 ? a.PX : (X.ax == 1 ? a.PY : PZ);
+  X.P1 = X.a1 == 0 ? a.PX : a.PY;
+  X.P2 = X.a2 == 1 ? a.PY : PZ;
+  const uint32_t sy = (uint32_t)a.PX, sz = (uint32_t)a.PX * (uint32_t)a.PY;
+  X.gsd = X.ax == 0 ? 1u : (X.ax == 1 ? sy : sz);
+  X.gs1 = X.a1 == 0 ? 1u : sy;
+  X.gs2 = X.a2 == 1 ? sy : sz;
+  X.nslabs = (X.PD - 4 + 1) / S + 1;  // slabs [1 + b S, 1 + (b+1) S) cover [1, n + 2]
+  return X;
 }
 
-// Block-uniform box of slab m (traversal order) and this ray's k range in it.
-struct SlabBox {
-  int kin, kout;
-  int dlo, e1lo, e2lo, E1, E2;
-};
+// One grid ray (view record V, pixel (jc, ic)) into G.r1/r2[v][p]; point 0 also stores the view's
+// source and margin. Returns false if the ray does not advance along the dominant axis: the slice
+// of a tile's frustum between two slab planes is the hull of its corner rays' intersections only
+// if every ray of the tile crosses the planes forwards; a flat tile's directions are affine in
+// (j, i), so it suffices that each corner ray does (a corner ray that misses the volume is not
+// checked by its thread's own test). 0.2 matches that per-ray test.
+__device__ __forceinline__ bool grid_ray(QuadGrid& G, const FpArgs& a, const SlabAxes& X,
+                                         const double* V, int v, int p, double jc, double ic) {
+  auto pickd = [](int k, double x, double y, double z) { return k == 0 ? x : (k == 1 ? y : z); };
+  double dx, dy, dz;
+  ray_dir(V, jc, ic, &dx, &dy, &dz);
+  const double dd = pickd(X.ax, dx, dy, dz);
+  const double rd = 1.0 / dd;
+  G.r1[v][p] = (float)(pickd(X.a1, dx, dy, dz) * rd);
+  G.r2[v][p] = (float)(pickd(X.a2, dx, dy, dz) * rd);
+  if (p == 0) {
+    const double sx = V[0] - a.ox + 2.0, sy = V[1] - a.oy + 2.0, sz = V[2] - a.oz + 2.0;
+    G.src[v][0] = (float)pickd(X.ax, sx, sy, sz);
+    G.src[v][1] = (float)pickd(X.a1, sx, sy, sz);
+    G.src[v][2] = (float)pickd(X.a2, sx, sy, sz);
+    G.src[v][3] = V[24] == (double)FC_DET_CYLINDER ? 2.0f : 1.0f;
+  }
+  return dd * (double)X.dir / sqrt(dx * dx + dy * dy + dz * dz) > 0.2;
+}
 
+// Union over views v0 .. v0 + nv - 1 of the sub-tile's frustum slice between slab m's planes
+// (traversal order), widened by the interpolation margin and clipped to the padded volume:
+// (e1lo, e2lo, E1, E2).
 template <int S>
-__device__ __forceinline__ SlabBox slab_box(const SlabPlan& P, int m) {
-  SlabBox B;
-  const int b = P.dir > 0.0f ? m : P.nslabs - 1 - m;
+__device__ __forceinline__ int4 slab_box(const QuadGrid& G, const SlabAxes& X, int v0, int nv,
+                                         int cj0, int ci0, int cw, int m) {
+  const int b = X.dir > 0.0f ? m : X.nslabs - 1 - m;
   const float lo = 1.0f + (float)(b * S), hi = lo + (float)S;
-  const float Pin = P.dir > 0.0f ? lo : hi, Pout = P.dir > 0.0f ? hi : lo;
-  int kin = (int)ceilf((Pin - P.pd0) * P.inv_sd);
-  int kout = (int)ceilf((Pout - P.pd0) * P.inv_sd);
-  B.kin = min(max(kin, 0), P.nk);
-  B.kout = min(max(kout, 0), P.nk);
-  float mn1 = 3e38f, mx1 = -3e38f, mn2 = 3e38f, mx2 = -3e38f;
-  const float tlo = lo - P.src_d, thi = hi - P.src_d;
+  float mn1 = 3e38f, mx1 = -3e38f, mn2 = 3e38f, mx2 = -3e38f, mg = 0.0f;
+  for (int v = v0; v < v0 + nv; ++v) {
+    const float tlo = lo - G.src[v][0], thi = hi - G.src[v][0];
+    const float s1 = G.src[v][1], s2 = G.src[v][2];
+    mg = fmaxf(mg, G.src[v][3]);
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    float u1 = fmaf(tlo, P.rc1[c], P.src_1), v1 = fmaf(thi, P.rc1[c], P.src_1);
-    float u2 = fmaf(tlo, P.rc2[c], P.src_2), v2 = fmaf(thi, P.rc2[c], P.src_2);
-    mn1 = fminf(mn1, fminf(u1, v1)); mx1 = fmaxf(mx1, fmaxf(u1, v1));
-    mn2 = fminf(mn2, fminf(u2, v2)); mx2 = fmaxf(mx2, fmaxf(u2, v2));
+    for (int c = 0; c < 4; ++c) {
+      const int p = (ci0 + (c >> 1) * (1 + cw)) * 4 + cj0 + (c & 1) * (1 + cw);
+      const float q1 = G.r1[v][p], q2 = G.r2[v][p];
+      float u1 = fmaf(tlo, q1, s1), w1 = fmaf(thi, q1, s1);
+      float u2 = fmaf(tlo, q2, s2), w2 = fmaf(thi, q2, s2);
+      mn1 = fminf(mn1, fminf(u1, w1)); mx1 = fmaxf(mx1, fmaxf(u1, w1));
+      mn2 = fminf(mn2, fminf(u2, w2)); mx2 = fmaxf(mx2, fmaxf(u2, w2));
+    }
   }
   // clamp in float first (rays nearly parallel to a slab give huge values)
-  const float P1f = (float)P.P1, P2f = (float)P.P2;
+  const float P1f = (float)X.P1, P2f = (float)X.P2;
   mn1 = fminf(fmaxf(mn1, -4.0f), P1f + 4.0f); mx1 = fminf(fmaxf(mx1, -4.0f), P1f + 4.0f);
   mn2 = fminf(fmaxf(mn2, -4.0f), P2f + 4.0f); mx2 = fminf(fmaxf(mx2, -4.0f), P2f + 4.0f);
-  B.e1lo = max((int)floorf(mn1 - P.margin), 0);
-  const int e1hi = min((int)floorf(mx1 + P.margin) + 1, P.P1 - 1);
-  B.e2lo = max((int)floorf(mn2 - P.margin), 0);
-  const int e2hi = min((int)floorf(mx2 + P.margin) + 1, P.P2 - 1);
-  B.E1 = e1hi - B.e1lo + 1;
-  B.E2 = e2hi - B.e2lo + 1;
-  B.dlo = (int)lo - 1;
-  return B;
+  const int e1lo = max((int)floorf(mn1 - mg), 0);
+  const int e1hi = min((int)floorf(mx1 + mg) + 1, X.P1 - 1);
+  const int e2lo = max((int)floorf(mn2 - mg), 0);
+  const int e2hi = min((int)floorf(mx2 + mg) + 1, X.P2 - 1);
+  return make_int4(e1lo, e2lo, e1hi - e1lo + 1, e2hi - e2lo + 1);
 }
 
 // Trilinear sample from an LDS brick laid out dominant axis fastest (pitch DP), then minor axis 1
@@ -355,27 +358,14 @@ __device__ __forceinline__ float lds_trilinear(const char* base, uint32_t E1u, u
   return lerp(c0, c1, a2f);
 }
 
+// Sum of this thread's ray samples, slab by slab through LDS bricks whose boxes come from G
+// (views v0 .. v0 + nv - 1, sub-tile (cj0, ci0, cw)), or from global memory when `fallback`.
 template <int DT, int S>
-__global__ __launch_bounds__(256) void fp_lds_kernel(FpArgs a) {
+__device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
+                                       const QuadGrid& G, const SlabAxes& X, int v0, int nv,
+                                       int cj0, int ci0, int cw, const RaySeg& r, bool fallback,
+                                       int lds_cap) {
   constexpr int DP = S + 3;  // dominant-axis pitch of the LDS brick
-  __shared__ float brick[kLdsCap];
-  int t, j0, i0;
-  fp_tile_origin(a.tiles_w, a.tiles_per_view, a.group, a.nviews, &t, &j0, &i0);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int j = j0 + (wave & 1) * 8 + (lane & 7);
-  const int i = i0 + (wave >> 1) * 8 + (lane >> 3);
-  const bool pix = (t < a.nviews) && (j < a.nw) && (i < a.nh);
-  const double* V = a.views + (size_t)min(t, a.nviews - 1) * FC_VIEW_DOUBLES;
-  RaySeg r;
-  r.nk = 0;
-  r.px = r.py = r.pz = r.sx = r.sy = r.sz = 0.f;
-  if (pix) r = ray_segment(V, j, i, a.ox, a.oy, a.oz, a.nx, a.ny, a.nz, a.h, a.max_steps, 2.0f);
-  SlabPlan P;
-  make_plan<S>(P, V, r, j0, i0, a.nw, a.nh, a.ox, a.oy, a.oz, a.PX, a.PY, a.nz);
-  // every active ray must advance along the dominant axis in the same direction
-  const bool bad = (r.nk > 0 && !(P.sd * P.dir > 0.2f * (float)a.h)) || !P.corners_ok;
-  const bool fallback = __syncthreads_or(bad);
-
   float acc = 0.0f;
   if (fallback) {
     for (int k = 0; k < r.nk; ++k) {
@@ -383,101 +373,247 @@ __global__ __launch_bounds__(256) void fp_lds_kernel(FpArgs a) {
       acc += trilinear_sample<DT, FP_PLAIN>(a, fmaf(kf, r.sx, r.px), fmaf(kf, r.sy, r.py),
                                             fmaf(kf, r.sz, r.pz));
     }
-  } else {
-    for (int m = 0; m < P.nslabs; ++m) {
-      const SlabBox B = slab_box<S>(P, m);
-      if (B.E1 <= 0 || B.E2 <= 0) continue;  // the tile's frustum misses the block here
-      const int PS = DP * B.E1;
-      const int NE = PS * B.E2;
-      if (NE > a.lds_cap) {
-        for (int k = B.kin; k < B.kout; ++k) {
-          float kf = (float)k;
-          acc += trilinear_sample<DT, FP_PLAIN>(a, fmaf(kf, r.sx, r.px), fmaf(kf, r.sy, r.py),
-                                                fmaf(kf, r.sz, r.pz));
-        }
-        continue;
+    return acc;
+  }
+  auto pick = [](int k, float x, float y, float z) { return k == 0 ? x : (k == 1 ? y : z); };
+  const float pd0 = pick(X.ax, r.px, r.py, r.pz), p10 = pick(X.a1, r.px, r.py, r.pz),
+              p20 = pick(X.a2, r.px, r.py, r.pz);
+  const float sd = pick(X.ax, r.sx, r.sy, r.sz), s1 = pick(X.a1, r.sx, r.sy, r.sz),
+              s2 = pick(X.a2, r.sx, r.sy, r.sz);
+  const float inv_sd = 1.0f / sd;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int m = 0; m < X.nslabs; ++m) {
+    if (m % kSlabTab == 0) {
+      __syncthreads();  // the previous table is no longer read
+      const int mm = m + (int)threadIdx.x;
+      if (threadIdx.x < kSlabTab && mm < X.nslabs)
+        tab[threadIdx.x] = slab_box<S>(G, X, v0, nv, cj0, ci0, cw, mm);
+      __syncthreads();
+    }
+    const int4 B = tab[m % kSlabTab];
+    const int E1 = B.z, E2 = B.w;
+    const int b = X.dir > 0.0f ? m : X.nslabs - 1 - m;
+    const float lo = 1.0f + (float)(b * S), hi = lo + (float)S;
+    const float Pin = X.dir > 0.0f ? lo : hi, Pout = X.dir > 0.0f ? hi : lo;
+    const int kin = min(max((int)ceilf((Pin - pd0) * inv_sd), 0), r.nk);
+    const int kout = min(max((int)ceilf((Pout - pd0) * inv_sd), 0), r.nk);
+    if (E1 <= 0 || E2 <= 0) continue;  // the frustum misses the block here
+    const int dlo = (int)lo - 1;
+    const int PS = DP * E1;
+    const int NE = PS * E2;
+    if (NE > lds_cap) {
+      for (int k = kin; k < kout; ++k) {
+        float kf = (float)k;
+        acc += trilinear_sample<DT, FP_PLAIN>(a, fmaf(kf, r.sx, r.px), fmaf(kf, r.sy, r.py),
+                                              fmaf(kf, r.sz, r.pz));
       }
-      __syncthreads();  // the previous brick is no longer read
-      // Fill. Dominant axis x (contiguous in HBM and in the brick): each lane keeps one offset d
-      // along x and walks rows, so 11 consecutive lanes read one row's consecutive x and an
-      // element costs one row decomposition; four rows' loads are issued before their LDS
-      // writes. Dominant axis y or z: one thread per (minor1, minor2) column, minor axis 1
-      // being x so a wave's loads stay coalesced, and the DP
-      // elements along the dominant axis unrolled (LDS index d + DP c, pitch DP odd so the
-      // writes are bank-conflict free). Planes past the padded block (last slab) read its last
-      // plane and are never sampled.
-      const uint32_t gbase = mad_u24((uint32_t)B.dlo, P.gsd,
-                                     mad_u24((uint32_t)B.e1lo, P.gs1, (uint32_t)B.e2lo * P.gs2));
-      const uint32_t dmax = (uint32_t)(P.PD - 1 - B.dlo);  // >= 2: dlo = b S <= PD - 3
-      if (P.gsd == 1u) {
-        // lanes 0..54 of each wave: d = lane % DP (dominant offset) and row phase lane / DP; the
-        // block's 4 waves cover 20 consecutive rows per step
-        const float invE1 = 1.0f / (float)B.E1;
-        constexpr int kLanes = (64 / DP) * DP, kRowsW = 64 / DP, kRowsB = 4 * kRowsW;
-        if (lane < kLanes) {
-          const uint32_t rr = (uint32_t)(((float)lane + 0.5f) * (1.0f / (float)DP));
-          const uint32_t d = (uint32_t)lane - mul_u24(rr, (uint32_t)DP);
-          const uint32_t gd = gbase + min(d, dmax);
-          const int nrow = B.E1 * B.E2;
+      continue;
+    }
+    __syncthreads();  // the previous brick is no longer read
+    // Fill. Dominant axis x (contiguous in HBM and in the brick): each lane keeps one offset d
+    // along x and walks rows, so 11 consecutive lanes read one row's consecutive x and an
+    // element costs one row decomposition; four rows' loads are issued before their LDS
+    // writes. Dominant axis y or z: one thread per (minor1, minor2) column, minor axis 1
+    // being x so a wave's loads stay coalesced, and the DP elements along the dominant axis
+    // unrolled (LDS index d + DP c, pitch DP odd so the writes are bank-conflict free). Planes
+    // past the padded block (last slab) read its last plane and are never sampled.
+    const uint32_t gbase = mad_u24((uint32_t)dlo, X.gsd,
+                                   mad_u24((uint32_t)B.x, X.gs1, (uint32_t)B.y * X.gs2));
+    const uint32_t dmax = (uint32_t)(X.PD - 1 - dlo);  // >= 2: dlo = b S <= PD - 3
+    if (X.gsd == 1u) {
+      // lanes 0..54 of each wave: d = lane % DP (dominant offset) and row phase lane / DP; the
+      // block's 4 waves cover 20 consecutive rows per step
+      const float invE1 = 1.0f / (float)E1;
+      constexpr int kLanes = (64 / DP) * DP, kRowsW = 64 / DP, kRowsB = 4 * kRowsW;
+      if (lane < kLanes) {
+        const uint32_t rr = (uint32_t)(((float)lane + 0.5f) * (1.0f / (float)DP));
+        const uint32_t d = (uint32_t)lane - mul_u24(rr, (uint32_t)DP);
+        const uint32_t gd = gbase + min(d, dmax);
+        const int nrow = E1 * E2;
 #pragma unroll 1
-          for (int r0 = kRowsW * wave + (int)rr; r0 < nrow; r0 += 4 * kRowsB) {
-            uint32_t v[4];
+        for (int r0 = kRowsW * wave + (int)rr; r0 < nrow; r0 += 4 * kRowsB) {
+          uint32_t v[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              const int r = r0 + q * kRowsB;
-              v[q] = 0u;
-              if (r < nrow) {
-                const uint32_t m2 = (uint32_t)(((float)r + 0.5f) * invE1);
-                const uint32_t m1 = (uint32_t)r - mul_u24(m2, (uint32_t)B.E1);
-                v[q] = load_bits<DT>(a.data, mad_u24(m1, P.gs1, mad_u24(m2, P.gs2, gd)));
-              }
-            }
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              const int r = r0 + q * kRowsB;
-              if (r < nrow) brick[d + DP * (uint32_t)r] = bits_to_f32<DT>(v[q]);
+          for (int q = 0; q < 4; ++q) {
+            const int rw = r0 + q * kRowsB;
+            v[q] = 0u;
+            if (rw < nrow) {
+              const uint32_t m2 = (uint32_t)(((float)rw + 0.5f) * invE1);
+              const uint32_t m1 = (uint32_t)rw - mul_u24(m2, (uint32_t)E1);
+              v[q] = load_bits<DT>(a.data, mad_u24(m1, X.gs1, mad_u24(m2, X.gs2, gd)));
             }
           }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int rw = r0 + q * kRowsB;
+            if (rw < nrow) brick[d + DP * (uint32_t)rw] = bits_to_f32<DT>(v[q]);
+          }
         }
-      } else {
-        const int ncol = B.E1 * B.E2;
-        const float invE1 = 1.0f / (float)B.E1;
+      }
+    } else {
+      const int ncol = E1 * E2;
+      const float invE1 = 1.0f / (float)E1;
 #pragma unroll 1
-        for (int c = threadIdx.x; c < ncol; c += 256) {
-          const uint32_t m2 = (uint32_t)(((float)c + 0.5f) * invE1);  // exact: c < 2^10
-          const uint32_t m1 = (uint32_t)c - mul_u24(m2, (uint32_t)B.E1);
-          const uint32_t g = mad_u24(m1, P.gs1, mad_u24(m2, P.gs2, gbase));
-          float v[DP];
+      for (int c = threadIdx.x; c < ncol; c += 256) {
+        const uint32_t m2 = (uint32_t)(((float)c + 0.5f) * invE1);  // exact: c < 2^10
+        const uint32_t m1 = (uint32_t)c - mul_u24(m2, (uint32_t)E1);
+        const uint32_t g = mad_u24(m1, X.gs1, mad_u24(m2, X.gs2, gbase));
+        float v[DP];
 #pragma unroll
-          for (int d = 0; d < DP; ++d)
-            v[d] = load_elem<DT>(a.data, mad_u24(min((uint32_t)d, dmax), P.gsd, g));
+        for (int d = 0; d < DP; ++d)
+          v[d] = load_elem<DT>(a.data, mad_u24(min((uint32_t)d, dmax), X.gsd, g));
 #pragma unroll
-          for (int d = 0; d < DP; ++d) brick[d + DP * c] = v[d];
-        }
+        for (int d = 0; d < DP; ++d) brick[d + DP * c] = v[d];
       }
-      __syncthreads();
-      // Brick-relative LDS byte offset of a sample's corner (0,0,0). Sample coordinates are >= 0
-      // in the padded frame (ray_segment guarantees it), so truncation is floor; the 24-bit
-      // multiply-adds stay full rate.
-      const uint32_t E1u = (uint32_t)B.E1;
-      const uint32_t boff = (uint32_t)(B.dlo + DP * (B.e1lo + B.E1 * B.e2lo));
-      const uint32_t zbytes = 4u * (uint32_t)PS;  // one m2 plane
-      const char* base = reinterpret_cast<const char*>(brick) - 4u * boff;
-      auto sample = [&](float kf) {
-        return lds_trilinear<DP>(base, E1u, zbytes, fmaf(kf, P.sd, P.pd0),
-                                 fmaf(kf, P.s1, P.p10), fmaf(kf, P.s2, P.p20));
-      };
-      // two samples per iteration: both samples' LDS reads are in flight together
-      float kf = (float)B.kin;
-      int k = B.kin;
-      for (; k + 1 < B.kout; k += 2, kf += 2.0f) {
-        const float r0 = sample(kf), r1 = sample(kf + 1.0f);
-        acc += r0;
-        acc += r1;
+    }
+    __syncthreads();
+    // Brick-relative LDS byte offset of a sample's corner (0,0,0). Sample coordinates are >= 0
+    // in the padded frame (ray_segment guarantees it), so truncation is floor; the 24-bit
+    // multiply-adds stay full rate.
+    const uint32_t E1u = (uint32_t)E1;
+    const uint32_t boff = (uint32_t)(dlo + DP * (B.x + E1 * B.y));
+    const uint32_t zbytes = 4u * (uint32_t)PS;  // one m2 plane
+    const char* base = reinterpret_cast<const char*>(brick) - 4u * boff;
+    auto sample = [&](float kf) {
+      return lds_trilinear<DP>(base, E1u, zbytes, fmaf(kf, sd, pd0), fmaf(kf, s1, p10),
+                               fmaf(kf, s2, p20));
+    };
+    // two samples per iteration: both samples' LDS reads are in flight together
+    float kf = (float)kin;
+    int k = kin;
+    for (; k + 1 < kout; k += 2, kf += 2.0f) {
+      const float q0 = sample(kf), q1 = sample(kf + 1.0f);
+      acc += q0;
+      acc += q1;
+    }
+    if (k < kout) acc += sample(kf);
+  }
+  return acc;
+}
+
+// Shared-mode quads. Blocks (quad, quarter) decide the mode and record it; single-mode quads are
+// left to fp_lds_kernel. `mode` = 0: cost decision, 1: always shared.
+template <int DT, int S>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8)))
+void fp_quad_kernel(FpArgs a, uint8_t* __restrict__ quad_mode, int mode) {
+  constexpr int DP = S + 3;
+  __shared__ float brick[kQuadCap];
+  __shared__ int4 tab[kSlabTab];
+  __shared__ QuadGrid G;
+  __shared__ unsigned long long cost[2];
+  const int nvq = (a.nviews + kQuad - 1) / kQuad * kQuad;
+  int tg, j0, i0;
+  fp_tile_origin(a.tiles_w, a.tiles_per_view, a.group, nvq, &tg, &j0, &i0);
+  const int sub = tg % kQuad;
+  const int t0 = tg - sub;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const SlabAxes X =
+      slab_axes<S>(a, a.views + (size_t)min(t0, a.nviews - 1) * FC_VIEW_DOUBLES, j0, i0);
+  // the quad's 4x4 ray grid in each view, one thread per (view, point)
+  bool bad = false;
+  if (threadIdx.x < 16 * kQuad) {
+    const int v = threadIdx.x >> 4, p = threadIdx.x & 15, cj = p & 3, ci = p >> 2;
+    bad = !grid_ray(G, a, X, a.views + (size_t)min(t0 + v, a.nviews - 1) * FC_VIEW_DOUBLES, v, p,
+                    (double)min(j0 + (cj >> 1) * 8 + (cj & 1) * 7, a.nw - 1),
+                    (double)min(i0 + (ci >> 1) * 8 + (ci & 1) * 7, a.nh - 1));
+  }
+  if (threadIdx.x < 2) cost[threadIdx.x] = 0ull;
+  const bool quad_bad = __syncthreads_or(bad);
+  // items (kind, slab), kind 0-3: the quarters over all four views, 4-7: each view's whole
+  // tile, over every fourth slab (brick sizes change slowly from slab to slab); consecutive
+  // lanes take consecutive slabs of one kind; a brick over its budget counts 4x. Sums stay in
+  // registers and are reduced across the wave, so each wave adds once to each LDS total.
+  if (!quad_bad && mode == 0) {
+    unsigned long long csh = 0ull, csi = 0ull;
+    const int nsub = (X.nslabs + 3) / 4;
+    for (int w = threadIdx.x; w < 8 * nsub; w += 256) {
+      const int kind = w / nsub, m = 4 * (w - kind * nsub);
+      const int4 B = kind < 4 ? slab_box<S>(G, X, 0, kQuad, (kind & 1) * 2, (kind >> 1) * 2, 0, m)
+                              : slab_box<S>(G, X, kind - 4, 1, 0, 0, 2, m);
+      if (B.z > 0 && B.w > 0) {
+        const unsigned long long ne = (unsigned long long)DP * (unsigned)B.z * (unsigned)B.w;
+        const int cap = kind < 4 ? min(a.lds_cap, kQuadCap) : a.lds_cap;
+        const unsigned long long c = ne > (unsigned long long)cap ? 4ull * ne : ne;
+        if (kind < 4) csh += c; else csi += c;
       }
-      if (k < B.kout) acc += sample(kf);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      csh += __shfl_xor(csh, off);
+      csi += __shfl_xor(csi, off);
+    }
+    if (lane == 0) {
+      atomicAdd(&cost[0], csh);
+      atomicAdd(&cost[1], csi);
     }
   }
+  __syncthreads();
+  // A quad whose grid rays do not all advance along the dominant axis goes to fp_lds_kernel,
+  // which checks each view's tile on its own, and so does the last quad when the view count is
+  // not a multiple of four (shared mode would leave the waves of the missing views idle).
+  const bool shared = mode == 1 || (!quad_bad && t0 + kQuad <= a.nviews && cost[0] <= cost[1]);
+  const int tile = (i0 / kTile) * a.tiles_w + j0 / kTile;
+  if (sub == 0 && threadIdx.x == 0)
+    quad_mode[(size_t)(t0 / kQuad) * a.tiles_per_view + tile] = shared ? kQuadShared : 0;
+  if (!shared) return;
+  const int t = t0 + wave;
+  const int j = j0 + (sub & 1) * 8 + (lane & 7);
+  const int i = i0 + (sub >> 1) * 8 + (lane >> 3);
+  const bool pix = (t < a.nviews) && (j < a.nw) && (i < a.nh);
+  RaySeg r;
+  r.nk = 0;
+  r.px = r.py = r.pz = r.sx = r.sy = r.sz = 0.f;
+  if (pix)
+    r = ray_segment(a.views + (size_t)t * FC_VIEW_DOUBLES, j, i, a.ox, a.oy, a.oz, a.nx, a.ny,
+                    a.nz, a.h, a.max_steps, 2.0f);
+  const float sdir = X.ax == 0 ? r.sx : (X.ax == 1 ? r.sy : r.sz);
+  // every active ray must advance along the dominant axis in the same direction
+  const bool fallback = __syncthreads_or(quad_bad || (r.nk > 0 && !(sdir * X.dir > 0.2f * (float)a.h)));
+  const float acc = march<DT, S>(a, brick, tab, G, X, 0, kQuad, (sub & 1) * 2, (sub >> 1) * 2, 0,
+                                 r, fallback, min(a.lds_cap, kQuadCap));
+  if (pix) {
+    float scale = (a.norm ? a.norm[t] : 1.0f) * (float)a.h;
+    size_t idx = ((size_t)t * a.nh + i) * a.nw + j;
+    a.proj[idx] += scale * acc;
+  }
+}
+
+// Single-mode quads (all quads when quad_mode is null): the block takes the whole 16x16 tile of
+// view tg.
+template <int DT, int S>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8)))
+void fp_lds_kernel(FpArgs a, const uint8_t* __restrict__ quad_mode) {
+  __shared__ float brick[kLdsCap];
+  __shared__ int4 tab[kSlabTab];
+  __shared__ QuadGrid G;
+  const int nvq = (a.nviews + kQuad - 1) / kQuad * kQuad;
+  int t, j0, i0;
+  fp_tile_origin(a.tiles_w, a.tiles_per_view, a.group, nvq, &t, &j0, &i0);
+  if (quad_mode) {
+    const int tile = (i0 / kTile) * a.tiles_w + j0 / kTile;
+    if (quad_mode[(size_t)(t / kQuad) * a.tiles_per_view + tile] == kQuadShared) return;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j = j0 + (wave & 1) * 8 + (lane & 7);
+  const int i = i0 + (wave >> 1) * 8 + (lane >> 3);
+  const bool pix = (t < a.nviews) && (j < a.nw) && (i < a.nh);
+  const double* V = a.views + (size_t)min(t, a.nviews - 1) * FC_VIEW_DOUBLES;
+  const SlabAxes X = slab_axes<S>(a, V, j0, i0);
+  // the tile's four corner rays (grid points 0, 3, 12, 15 of view slot 0)
+  bool bad = false;
+  if (threadIdx.x < 4) {
+    const int c = threadIdx.x, p = (c >> 1) * 12 + (c & 1) * 3;
+    bad = !grid_ray(G, a, X, V, 0, p, (double)min(j0 + (c & 1) * (kTile - 1), a.nw - 1),
+                    (double)min(i0 + (c >> 1) * (kTile - 1), a.nh - 1));
+  }
+  RaySeg r;
+  r.nk = 0;
+  r.px = r.py = r.pz = r.sx = r.sy = r.sz = 0.f;
+  if (pix) r = ray_segment(V, j, i, a.ox, a.oy, a.oz, a.nx, a.ny, a.nz, a.h, a.max_steps, 2.0f);
+  const float sdir = X.ax == 0 ? r.sx : (X.ax == 1 ? r.sy : r.sz);
+  // every active ray must advance along the dominant axis in the same direction
+  const bool fallback = __syncthreads_or(bad || (r.nk > 0 && !(sdir * X.dir > 0.2f * (float)a.h)));
+  const float acc = march<DT, S>(a, brick, tab, G, X, 0, 1, 0, 0, 2, r, fallback, a.lds_cap);
   if (pix) {
     float scale = (a.norm ? a.norm[t] : 1.0f) * (float)a.h;
     size_t idx = ((size_t)t * a.nh + i) * a.nw + j;
@@ -986,6 +1122,11 @@ int launch_project(int device, const void* vol, int dt, const int vs[3], const d
   }
   FC_CHECK_ARG(PY <= 65535 && PZ <= 65535, "volume dimension too large for staging grid");
   FC_CHECK_ARG((size_t)PX * PY * PZ < (1ull << 32), "volume too large for 32-bit voxel offsets");
+  // one mode byte per quad of the LDS projector after the staged volume
+  const size_t flags_off = (bytes + 255) & ~(size_t)255;
+  const size_t nquads = (size_t)((ps[0] + kTile - 1) / kTile) * ((ps[1] + kTile - 1) / kTile) *
+                        (size_t)((ps[2] + kQuad - 1) / kQuad);
+  if (layout == FP_LDS) bytes = flags_off + nquads;
   void* ws = nullptr;
   int rc = acquire_ws(device, SLOT_OCTET, bytes, stream, &ws);
   if (rc) return rc;
@@ -1021,20 +1162,36 @@ int launch_project(int device, const void* vol, int dt, const int vs[3], const d
   // FASTCONE_FP_LDS_CAP lowers the LDS brick budget (testing the global-gather fallback)
   const char* cap = std::getenv("FASTCONE_FP_LDS_CAP");
   a.lds_cap = cap ? std::max(0, std::min(kLdsCap, std::atoi(cap))) : kLdsCap;
-  long long nblocks = (long long)a.tiles_per_view * a.nviews;
+  // The LDS projector deals quads of four views; `group` stays a multiple of four.
+  const int nvq = (a.nviews + kQuad - 1) / kQuad * kQuad;
+  const long long nblocks = (long long)a.tiles_per_view * (layout == FP_LDS ? nvq : a.nviews);
   FC_CHECK_ARG(nblocks < (1ll << 31), "too many projection tiles");
+  if (layout == FP_LDS) a.group = std::min((a.group + kQuad - 1) / kQuad * kQuad, nvq);
+  // FASTCONE_FP_QUAD = "never" / "always" forces single / shared mode for every quad (tests)
+  const char* qe = std::getenv("FASTCONE_FP_QUAD");
+  const int qmode = qe && std::strcmp(qe, "never") == 0 ? -1
+                    : (qe && std::strcmp(qe, "always") == 0 ? 1 : 0);
+  uint8_t* quad_mode = qmode < 0 ? nullptr : static_cast<uint8_t*>(ws) + flags_off;
   if (nblocks > 0) {
     int tok = prof_begin("project", stream);
+    const int nb = (int)nblocks;
     if (layout == FP_OCTET) {
-      if (dt == FC_F32) fp_kernel<FC_F32, FP_OCTET><<<(int)nblocks, 256, 0, stream>>>(a);
-      else if (dt == FC_BF16) fp_kernel<FC_BF16, FP_OCTET><<<(int)nblocks, 256, 0, stream>>>(a);
-      else fp_kernel<FC_F16, FP_OCTET><<<(int)nblocks, 256, 0, stream>>>(a);
+      if (dt == FC_F32) fp_kernel<FC_F32, FP_OCTET><<<nb, 256, 0, stream>>>(a);
+      else if (dt == FC_BF16) fp_kernel<FC_BF16, FP_OCTET><<<nb, 256, 0, stream>>>(a);
+      else fp_kernel<FC_F16, FP_OCTET><<<nb, 256, 0, stream>>>(a);
     } else if (layout == FP_PLAIN) {
-      fp_kernel<FC_F32, FP_PLAIN><<<(int)nblocks, 256, 0, stream>>>(a);
+      fp_kernel<FC_F32, FP_PLAIN><<<nb, 256, 0, stream>>>(a);
     } else {
-      if (dt == FC_F32) fp_lds_kernel<FC_F32, 8><<<(int)nblocks, 256, 0, stream>>>(a);
-      else if (dt == FC_BF16) fp_lds_kernel<FC_BF16, 8><<<(int)nblocks, 256, 0, stream>>>(a);
-      else fp_lds_kernel<FC_F16, 8><<<(int)nblocks, 256, 0, stream>>>(a);
+      if (qmode >= 0) {
+        if (dt == FC_F32) fp_quad_kernel<FC_F32, 8><<<nb, 256, 0, stream>>>(a, quad_mode, qmode);
+        else if (dt == FC_BF16) fp_quad_kernel<FC_BF16, 8><<<nb, 256, 0, stream>>>(a, quad_mode, qmode);
+        else fp_quad_kernel<FC_F16, 8><<<nb, 256, 0, stream>>>(a, quad_mode, qmode);
+      }
+      if (qmode <= 0) {
+        if (dt == FC_F32) fp_lds_kernel<FC_F32, 8><<<nb, 256, 0, stream>>>(a, quad_mode);
+        else if (dt == FC_BF16) fp_lds_kernel<FC_BF16, 8><<<nb, 256, 0, stream>>>(a, quad_mode);
+        else fp_lds_kernel<FC_F16, 8><<<nb, 256, 0, stream>>>(a, quad_mode);
+      }
     }
     prof_end(tok, stream);
     FC_HIP(hipGetLastError());

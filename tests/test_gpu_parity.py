@@ -27,10 +27,12 @@ def rel(a, b):
 
 
 def _setup(rng, n=12, vol=(18, 20, 22), vol_dim=(24, 24, 26), vol_start=(2, 3, 1),
-           det=(26, 30), det_dim=(30, 34), D=60.0, pw=0.9, mis=None):
+           det=(26, 30), det_dim=(30, 34), D=60.0, pw=0.9, mis=None, ang=None):
     """C-order vol=(nz,ny,nx), vol_dim=(Nz,Ny,Nx), vol_start=(z,y,x), det=(nh,nw)."""
     from fastcone_amd.geometry import Geometry
-    ang = np.sort(rng.uniform(0, 2 * np.pi, n))
+    if ang is None:
+        ang = np.sort(rng.uniform(0, 2 * np.pi, n))
+    n = len(ang)
     hmid = rng.uniform(vol_dim[0] / 2 - 3, vol_dim[0] / 2 + 3, n)
     woff = rng.integers(-3, 5, n)
     hoff = rng.integers(-3, 5, n)
@@ -58,14 +60,20 @@ def test_project_f32_parity(gpu, h, mis):
     assert rel(p, ref) < 5e-5
 
 
-@pytest.mark.parametrize("layout,cap", [("octet", None), ("plain", None), ("lds", None),
-                                        ("lds", "600"), ("lds", "0")])
-def test_project_layouts_agree(gpu, layout, cap, monkeypatch):
-    # cap lowers the LDS brick budget so some (600) or all (0) slabs take the global fallback
+@pytest.mark.parametrize("layout,cap,quad", [("octet", None, None), ("plain", None, None),
+                                             ("lds", None, None), ("lds", None, "always"),
+                                             ("lds", None, "never"), ("lds", "600", "always"),
+                                             ("lds", "600", "never"), ("lds", "0", None)])
+def test_project_layouts_agree(gpu, layout, cap, quad, monkeypatch):
+    # cap lowers the LDS brick budget so some (600) or all (0) slabs take the global fallback;
+    # quad forces every quad of views into shared (one brick per quarter tile for four views) or
+    # single (one view's whole tile per block) mode
     from fastcone_amd import ops
     monkeypatch.setenv("FASTCONE_FP_LAYOUT", layout)
     if cap is not None:
         monkeypatch.setenv("FASTCONE_FP_LDS_CAP", cap)
+    if quad is not None:
+        monkeypatch.setenv("FASTCONE_FP_QUAD", quad)
     rng = np.random.default_rng(12)
     g, prob = _setup(rng)
     x = rng.standard_normal(g.vol_shape).astype(np.float32)
@@ -74,17 +82,40 @@ def test_project_layouts_agree(gpu, layout, cap, monkeypatch):
     assert rel(p, ref) < 5e-5
 
 
+@pytest.mark.parametrize("quad", [None, "always", "never"])
 @pytest.mark.parametrize("seed", [14, 33])
-def test_project_lds_steep_and_wide_cone(gpu, seed):
+def test_project_lds_steep_and_wide_cone(gpu, seed, quad, monkeypatch):
     # short source distance and a tilted detector: large cone angles, views whose dominant axis
     # is z for some tiles, and bricks that exceed the LDS budget; seed 33 has a tile whose corner
     # ray (missing the volume) points against the tile's dominant axis
     from fastcone_amd import ops
+    if quad is not None:
+        monkeypatch.setenv("FASTCONE_FP_QUAD", quad)
     rng = np.random.default_rng(seed)
     g, prob = _setup(rng, D=28.0, pw=1.6, mis=[0.0, 0.0, 0.0, 0.9, 0.3, 0.2])
     x = rng.standard_normal(g.vol_shape).astype(np.float32)
     p = ops.project(torch.from_numpy(x).to(gpu), g, step=0.5).cpu()
     ref = oracle.project(prob, x, window=g.det_shape[::-1], h=0.5)
+    assert rel(p, ref) < 5e-5
+
+
+@pytest.mark.parametrize("quad", [None, "always", "never"])
+@pytest.mark.parametrize("nviews", [13, 16])
+def test_project_quads_dense_views(gpu, nviews, quad, monkeypatch):
+    # consecutive views 0.5 deg apart share their bricks (shared mode) unless a mode is forced;
+    # 13 views leave a quad with three padding views; the views straddle the 45 deg diagonal,
+    # where a tile's dominant axis can change within a quad
+    from fastcone_amd import ops
+    if quad is not None:
+        monkeypatch.setenv("FASTCONE_FP_QUAD", quad)
+    rng = np.random.default_rng(40 + nviews)
+    ang = np.deg2rad(41.0 + 0.5 * np.arange(nviews))
+    g, prob = _setup(rng, ang=ang, vol=(20, 24, 22), vol_dim=(26, 28, 26), det=(40, 44),
+                     det_dim=(44, 48))
+    x = rng.standard_normal(g.vol_shape).astype(np.float32)
+    norm = rng.uniform(0.5, 1.5, g.nviews).astype(np.float32)
+    p = ops.project(torch.from_numpy(x).to(gpu), g, step=0.5, norm=torch.from_numpy(norm)).cpu()
+    ref = oracle.project(prob, x, window=g.det_shape[::-1], h=0.5, norm=norm)
     assert rel(p, ref) < 5e-5
 
 
