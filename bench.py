@@ -148,7 +148,8 @@ def main():
     traffic = measured_traffic(nv)
     roofline = {
         "bound": "gather",
-        "kernel": "fp_lds_kernel (ray-march projector, LDS slab bricks)",
+        "kernel": "fp_quad_kernel + fp_lds_kernel (ray-march projector, LDS slab bricks shared by "
+                  "four-view quads)",
         "achieved": fp_rate / 1e9, "peak": peak_fp * 8 / 1e9, "unit": "Gtap/s",
         "frac": fp_rate / (peak_fp * 8),
         "traffic": None if traffic is None else traffic["bytes_per_launch"],
@@ -221,7 +222,7 @@ def measured_traffic(nv):
     """HBM bytes per projector launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     passes (each scaled by what the counter reads for a copy kernel of known size), when they
     were measured at this view count; None otherwise."""
-    path = os.path.join(ROOT, "profiles", "fp_lds_kernel_traffic.json")
+    path = os.path.join(ROOT, "profiles", "fp_quad_kernel_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:

@@ -494,7 +494,7 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
 // Shared-mode quads. Blocks (quad, quarter) decide the mode and record it; single-mode quads are
 // left to fp_lds_kernel. `mode` = 0: cost decision, 1: always shared.
 template <int DT, int S>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
 void fp_quad_kernel(FpArgs a, uint8_t* __restrict__ quad_mode, int mode) {
   constexpr int DP = S + 3;
   __shared__ float brick[kQuadCap];
