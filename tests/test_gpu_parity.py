@@ -119,6 +119,23 @@ def test_project_quads_dense_views(gpu, nviews, quad, monkeypatch):
     assert rel(p, ref) < 5e-5
 
 
+@pytest.mark.parametrize("quad", ["always", "never"])
+@pytest.mark.parametrize("dtype", ["bfloat16", "float16"])
+def test_project_16bit_storage_dense_views(gpu, dtype, quad, monkeypatch):
+    # 16-bit bricks filled for four views at once (shared quads) and one view at a time
+    from fastcone_amd import ops
+    monkeypatch.setenv("FASTCONE_FP_QUAD", quad)
+    dt = getattr(torch, dtype)
+    rng = np.random.default_rng(50)
+    ang = np.deg2rad(10.0 + 0.5 * np.arange(8))
+    g, prob = _setup(rng, ang=ang, vol=(20, 24, 22), vol_dim=(26, 28, 26), det=(40, 44),
+                     det_dim=(44, 48))
+    x = rng.standard_normal(g.vol_shape).astype(np.float32)
+    p = ops.project(torch.from_numpy(x).to(gpu).to(dt), g, step=0.5).cpu().numpy()
+    ref_rounded = oracle.project(prob, _round(x, dt), window=g.det_shape[::-1], h=0.5)
+    assert rel(p, ref_rounded) < 5e-5
+
+
 @pytest.mark.parametrize("dtype", ["bfloat16", "float16"])
 def test_project_16bit_storage(gpu, dtype):
     from fastcone_amd import ops
