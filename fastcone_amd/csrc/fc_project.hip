@@ -593,6 +593,7 @@ void fp_lds_kernel(FpArgs a, const uint8_t* __restrict__ quad_mode) {
     const int tile = (i0 / kTile) * a.tiles_w + j0 / kTile;
     if (quad_mode[(size_t)(t / kQuad) * a.tiles_per_view + tile] == kQuadShared) return;
   }
+  if (t >= a.nviews) return;  // a padding view of the last quad (whole block)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int j = j0 + (wave & 1) * 8 + (lane & 7);
   const int i = i0 + (wave >> 1) * 8 + (lane >> 3);

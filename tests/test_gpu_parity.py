@@ -119,6 +119,22 @@ def test_project_quads_dense_views(gpu, nviews, quad, monkeypatch):
     assert rel(p, ref) < 5e-5
 
 
+@pytest.mark.parametrize("quad", [None, "always", "never"])
+@pytest.mark.parametrize("nviews", [1, 2, 3, 5])
+def test_project_small_view_counts(gpu, nviews, quad, monkeypatch):
+    # stacks that leave one to three padding views in their last quad of four
+    from fastcone_amd import ops
+    if quad is not None:
+        monkeypatch.setenv("FASTCONE_FP_QUAD", quad)
+    rng = np.random.default_rng(60 + nviews)
+    ang = np.deg2rad(100.0 + 0.5 * np.arange(nviews))
+    g, prob = _setup(rng, ang=ang)
+    x = rng.standard_normal(g.vol_shape).astype(np.float32)
+    p = ops.project(torch.from_numpy(x).to(gpu), g, step=0.5).cpu()
+    ref = oracle.project(prob, x, window=g.det_shape[::-1], h=0.5)
+    assert rel(p, ref) < 5e-5
+
+
 @pytest.mark.parametrize("quad", ["always", "never"])
 @pytest.mark.parametrize("dtype", ["bfloat16", "float16"])
 def test_project_16bit_storage_dense_views(gpu, dtype, quad, monkeypatch):
