@@ -62,8 +62,9 @@ def log(*a):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on exactly one GPU; multi-GPU sharding is a "
-                                   "design (see DESIGN.md), --gpus must be 1"}))
+        print(json.dumps({"error": "the projector / backprojector runs on exactly one GPU; "
+                                   "sharding over several GPUs (views or z-slabs per GPU) is not "
+                                   "implemented, --gpus must be 1"}))
         sys.exit(2)
     import torch
     from fastcone_amd import _native, ops, phantom
