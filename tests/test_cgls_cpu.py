@@ -57,3 +57,25 @@ def test_cgls_warm_start_and_checkpoint_resume(tmp_path):
     xs = torch.from_numpy(scipy.linalg.lstsq(M, b)[0])
     _, info_w = cgls(A, At, bt, x0=xs, iters=3, atb_norm=float(torch.linalg.norm(At(bt))))
     assert info_w["residual"][0] < 1e-10
+
+
+def test_poisson_log_statistics():
+    # counts ~ Poisson(lam), lam = I0 exp(-s p): -ln(N / I0) / s has mean p + 1 / (2 lam s) and
+    # variance 1 / (lam s^2) to leading order in 1 / lam (delta method)
+    from fastcone_amd.cgls import poisson_log
+    I0, n = 1e4, 400_000
+    for p, s in ((1.0, 1.0), (0.4, 2.5)):
+        lam = I0 * np.exp(-s * p)
+        q = poisson_log(torch.full((n,), p), I0, scale=s, seed=7).double().numpy()
+        assert abs(q.mean() - (p + 1.0 / (2.0 * lam * s))) < 5.0 * np.sqrt(1.0 / (lam * s * s) / n)
+        assert abs(q.var() * lam * s * s - 1.0) < 0.02
+
+
+def test_poisson_log_zero_counts_and_seed():
+    from fastcone_amd.cgls import poisson_log
+    p = torch.tensor([60.0, 0.0, 1.0])
+    q = poisson_log(p, 1e4, seed=3)
+    assert q[0].item() == pytest.approx(np.log(1e4), rel=1e-6)  # zero counts clamp to one
+    assert torch.equal(poisson_log(p, 1e4, seed=3), q)
+    assert not torch.equal(poisson_log(torch.ones(1000), 1e4, seed=3),
+                           poisson_log(torch.ones(1000), 1e4, seed=4))

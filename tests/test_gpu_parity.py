@@ -344,6 +344,28 @@ def test_abi_misaligned_backproject(gpu):
     assert rel(vol, ref) < 5e-5
 
 
+def test_abi_fbp_matches_oracle_pipeline(gpu):
+    # fastcone.fbp = the reference's cosine pre-weight (pixel-index offsets, source row zOrigin),
+    # ramp filter and backprojection with the default kernelMode 2 (Feldkamp) and norm 1/N, on
+    # the whole detector; proj is filtered in place as in fastcone.py:260-270
+    from fastcone_amd import fastcone as cone
+    rng = np.random.default_rng(47)
+    nv, Nh, Nw = 9, 26, 30
+    Nz, Ny, Nx = 16, 18, 20
+    ang = np.sort(rng.uniform(0, 2 * np.pi, nv)).astype(np.float32)
+    z0 = rng.uniform(Nz / 2 - 2, Nz / 2 + 2, nv).astype(np.float32)
+    prob = oracle.Problem(ang.astype(np.float64), z0.astype(np.float64), 70.0, 0.8,
+                          vol_dim=(Nx, Ny, Nz), proj_dim=(Nw, Nh))
+    y = rng.standard_normal((nv, Nh, Nw)).astype(np.float32)
+    proj = y.copy()
+    vol = np.zeros((Nz, Ny, Nx), np.float32)
+    cone.fbp(vol, proj, ang, 70.0, z0, pixelWidth=0.8)
+    yf = oracle.rampfilter(y.astype(np.float64) * oracle.fbp_cosine_weight(y.shape, 70.0, z0))
+    ref = oracle.backproject(prob, yf, vol.shape, mode=2, norm=np.full(nv, 1.0 / nv))
+    assert rel(proj, yf) < 1e-5
+    assert rel(vol, ref) < 5e-5
+
+
 def test_abi_tiling_invariance(gpu):
     from fastcone_amd import fastcone as cone
     rng = np.random.default_rng(46)
