@@ -89,7 +89,7 @@ def main():
         vol, _ = phantom.config_phantom(1, device=dev)
     ang = 2 * np.pi * np.arange(nv) / nv
     geom = Geometry.circular(ang, D, PW, (N, N, N), (NDET, NDET))
-    weights = torch.from_numpy(ops.fdk_cosine_weights(geom, D, PW)).to(dev, torch.float32)
+    weights = ops.fdk_cosine_weights(geom, D, PW, device=dev)
     norm = torch.full((nv,), 1.0 / nv, dtype=torch.float32, device=dev)
     proj = torch.zeros(geom.proj_shape, dtype=torch.float32, device=dev)
     rec = torch.zeros(geom.vol_shape, dtype=torch.float32, device=dev)

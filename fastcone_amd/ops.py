@@ -206,29 +206,39 @@ def ramp_response(nw, device, dtype=torch.float32):
     return H.to(device=device, dtype=dtype)
 
 
-def fdk_cosine_weights(geom, D, pw):
-    """D / sqrt(D^2 + u^2 + v^2) with (u, v) the window pixel offsets from the detector centre at
-    the axis, in voxels (physical FDK pre-weight; the reference's fbp() uses pixel units)."""
-    v = geom.views
+def fdk_cosine_weights(geom, D, pw, device=None, chunk=32):
+    """FDK pre-weight of every window pixel: the cosine of the angle between its ray and the
+    detector normal, dperp / |P - src| (= D / sqrt(D^2 + u^2 + v^2) for an upright flat detector;
+    physical units, where the reference's fbp() uses pixel units). Flat-detector records.
+
+    Evaluated in FP64 on `device` (CPU by default), a chunk of views at a time, from six scalars per
+    view: |r0 + j du + i dv|^2 with r0 = d0 - src expands to a quadratic in (j, i). Returns an
+    FP32 tensor [nviews, nh, nw].
+    """
+    f64 = torch.float64
+    v = torch.as_tensor(np.ascontiguousarray(geom.views), dtype=f64, device=device)
+    nt = v.shape[0]
     nh, nw = geom.det_shape
-    # pixel centres relative to the detector centre, along the w / h unit vectors (flat records)
-    w = np.empty((geom.nviews, nh, nw))
-    src = v[:, 0:3]
-    d0 = v[:, 3:6]
-    du = v[:, 6:9]
-    dv = v[:, 9:12]
-    j = np.arange(nw, dtype=np.float64)
-    i = np.arange(nh, dtype=np.float64)
-    P = (d0[:, None, None, :] + j[None, None, :, None] * du[:, None, None, :]
-         + i[None, :, None, None] * dv[:, None, None, :])
-    r = P - src[:, None, None, :]
-    dist = np.linalg.norm(r, axis=-1)
-    # distance from the source to the detector plane along its normal
-    n = np.cross(du, dv)
-    n /= np.linalg.norm(n, axis=-1, keepdims=True)
-    dperp = np.abs(np.einsum("tk,tk->t", d0 - src, n))
-    w[:] = dperp[:, None, None] / dist
-    return w
+    out = torch.empty((nt, nh, nw), dtype=torch.float32, device=device)
+    if nt == 0:
+        return out
+    src, d0, du, dv = v[:, 0:3], v[:, 3:6], v[:, 6:9], v[:, 9:12]
+    r0 = d0 - src
+    nrm = torch.linalg.cross(du, dv, dim=1)
+    nrm = nrm / torch.linalg.norm(nrm, dim=1, keepdim=True)
+    dperp = (r0 * nrm).sum(1).abs()
+    c0 = (r0 * r0).sum(1)
+    cj, ci = 2.0 * (r0 * du).sum(1), 2.0 * (r0 * dv).sum(1)
+    cjj, cii, cij = (du * du).sum(1), (dv * dv).sum(1), 2.0 * (du * dv).sum(1)
+    j = torch.arange(nw, dtype=f64, device=device)[None, None, :]
+    i = torch.arange(nh, dtype=f64, device=device)[None, :, None]
+    for t0 in range(0, nt, chunk):
+        sl = slice(t0, min(nt, t0 + chunk))
+        e = lambda c: c[sl][:, None, None]
+        dist2 = (e(c0) + e(cj) * j + e(ci) * i + e(cjj) * j * j + e(cii) * i * i
+                 + e(cij) * i * j)
+        out[sl] = (e(dperp) / torch.sqrt(dist2)).to(torch.float32)
+    return out
 
 
 def fdk_filter(proj, geom, D, pw, out_dtype=torch.float32, weights=None):
@@ -240,8 +250,7 @@ def fdk_filter(proj, geom, D, pw, out_dtype=torch.float32, weights=None):
     _need_device(proj, "proj")
     nt, nh, nw = proj.shape
     if weights is None:
-        weights = torch.from_numpy(fdk_cosine_weights(geom, D, pw)).to(proj.device,
-                                                                      torch.float32)
+        weights = fdk_cosine_weights(geom, D, pw, device=proj.device)
     H = ramp_response(nw, proj.device) / pw
     q = proj.float() * weights
     spec = torch.fft.rfft(q, n=2 * nw, dim=2)

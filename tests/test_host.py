@@ -207,3 +207,25 @@ def test_windowed_ramp_filters():
     assert hi(c) < 0.5 * hi(plain)
     with pytest.raises(ValueError):
         fastcone.ramp_response(8, "hann")
+
+
+def test_fdk_cosine_weights_are_the_ray_normal_cosine():
+    # upright circular geometry: the detector normal is the horizontal direction from the source
+    # to the rotation axis, (cos a, sin a, 0) up to sign, fixed by the view angle alone
+    import torch
+    from fastcone_amd import ops
+    from fastcone_amd.geometry import Geometry
+    ang = np.array([0.0, 0.7, 2.9, 4.4])
+    g = Geometry.circular(ang, 40.0, 0.8, (10, 12, 14), (9, 11), hmid=[5.0, 4.0, 6.5, 5.0],
+                          woff=[0, 1, -2, 3], hoff=[0, 2, 1, -1], det_dim=(12, 14))
+    w = ops.fdk_cosine_weights(g, 40.0, 0.8, chunk=3).numpy()
+    v = g.views
+    nh, nw = g.det_shape
+    jj, ii = np.meshgrid(np.arange(nw), np.arange(nh))
+    for t, a in enumerate(ang):
+        P = v[t, 3:6] + jj[..., None] * v[t, 6:9] + ii[..., None] * v[t, 9:12]
+        r = P - v[t, 0:3]
+        n = np.array([np.cos(a), np.sin(a), 0.0])
+        ref = np.abs(r @ n) / np.linalg.norm(r, axis=-1)
+        assert np.allclose(w[t], ref, rtol=0, atol=2e-7)
+    assert w.max() <= 1.0 and w.min() > 0.5
