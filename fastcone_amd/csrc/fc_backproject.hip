@@ -265,7 +265,9 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
     const float umax = (float)Wu - 2.0f, vmax = (float)Wv - 2.0f;
     if (M[2] == 0.0 && M[10] == 0.0) {  // uniform per view
       // Columns independent of z: u and the weight are per column, the row is linear in z.
-      const double rc = 1.0 / C;
+      // FP32 reciprocal refined by one FP64 Newton step: relative error ~2^-44
+      double rc = (double)__builtin_amdgcn_rcpf((float)C);
+      rc = rc * fma(-C, rc, 2.0);
       const float u = clampf((float)(A * rc) - u0f, 0.0f, umax);
       const float vb = (float)(B * rc) - v0f;
       const float dv = (float)(M[6] * rc);
@@ -285,7 +287,7 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
 #pragma unroll
       for (int k = 0; k < kZR; ++k) {
         const float kf = (float)k;
-        const float rc = 1.0f / fmaf(kf, dc, cf);
+        const float rc = __builtin_amdgcn_rcpf(fmaf(kf, dc, cf));  // 1 ulp, as the positions
         const float u = clampf(fmaf(kf, da, af) * rc - u0f, 0.0f, umax);
         const float v = clampf(fmaf(kf, db, bf) * rc - v0f, 0.0f, vmax);
         float wt = s;
