@@ -223,8 +223,10 @@ __global__ __launch_bounds__(256) void fp_kernel(FpArgs a) {
 // bricks). When the views are far apart (sparse angular sampling) that union is wide, and the
 // quad is left to fp_lds_kernel, where each block takes the whole 16x16 tile of one view
 // (6 waves per SIMD, 24 KiB bricks). fp_quad_kernel decides per quad from the brick elements of
-// both layouts summed over the slabs (a brick over its budget counting 4x) and records the
-// decision in a byte per quad that fp_lds_kernel, launched next on the same stream, reads.
+// both layouts summed over every fourth slab (a brick over its budget counting 4x; the last quad
+// of a view count that is not a multiple of four, and a quad whose grid rays do not all advance
+// along the dominant axis, always go to fp_lds_kernel) and records the decision in a byte per
+// quad that fp_lds_kernel, launched next on the same stream, reads.
 // Bricks larger than the LDS budget (or tiles whose rays do not all advance along the dominant
 // axis) fall back to gathering from the staged volume in global memory, with the same arithmetic.
 
