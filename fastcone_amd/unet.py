@@ -137,3 +137,51 @@ class UNet3D(torch.nn.Module):
                 self.w[k].copy_(torch.from_numpy(z[f"w_{k}"]))
                 self.b[k].copy_(torch.from_numpy(z[f"b_{k}"]))
         return self
+
+
+def bench(batch=4, train_n=128, infer_n=256, warmup=6, steps=3, log=print):
+    """Times a training step (Adam, L2 loss, batch `batch` of train_n^3 random volumes) and one
+    inference at infer_n^3 on the current GPU; returns (step ms, inference ms)."""
+    import time
+    dev = torch.device("cuda", torch.cuda.current_device())
+    net = UNet3D(seed=0).to(dev)
+    opt = torch.optim.Adam(net.parameters(), lr=1e-3)
+    g = torch.Generator(device=dev).manual_seed(0)
+    x = torch.rand((batch,) + (train_n,) * 3, device=dev, generator=g) * 0.02
+    y = x + 0.001 * torch.randn(x.shape, device=dev, generator=g)
+
+    def train_step():
+        loss = ((net(x) - y) ** 2).mean() * UNet3D.SCALE ** 2
+        opt.zero_grad(set_to_none=True)
+        loss.backward()
+        opt.step()
+
+    for _ in range(warmup):
+        train_step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        train_step()
+    torch.cuda.synchronize()
+    step_ms = (time.perf_counter() - t0) / steps * 1e3
+    with torch.no_grad():
+        v = torch.rand((infer_n,) * 3, device=dev, generator=g) * 0.02
+        net(v)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        net(v)
+        torch.cuda.synchronize()
+    infer_ms = (time.perf_counter() - t0) * 1e3
+    log(f"train step (batch {batch}, {train_n}^3): {step_ms:.1f} ms; "
+        f"inference ({infer_n}^3): {infer_ms:.1f} ms")
+    return step_ms, infer_ms
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser(description="time the warm-start U-Net on the GPU")
+    ap.add_argument("--bench", action="store_true", help="time a training step and an inference")
+    ap.add_argument("--batch", type=int, default=4)
+    args = ap.parse_args()
+    if args.bench:
+        bench(batch=args.batch)
