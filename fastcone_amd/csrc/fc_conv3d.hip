@@ -104,7 +104,9 @@ __global__ __launch_bounds__(256) void flip_transpose_weights(const uint16_t* __
 }
 
 // ---- weight gradient -------------------------------------------------------------------------
-// grid.x: groups of ROWS x rows (n, z, y); grid.y: blocks of 16 k. dW (FP32) is accumulated.
+// grid.x: groups of kRows x-rows (n, z, y); grid.y: blocks of 64 k, one 16-column slice per wave.
+// The four waves read the same output-gradient values (served by L1 after the first wave), so each
+// is fetched from L2 once per 64 columns of K instead of once per 16. dW (FP32) is accumulated.
 constexpr int kRows = 8;
 template <int CIN, int COUT>
 __global__ __launch_bounds__(256) void conv_wgrad_mfma(const uint16_t* __restrict__ x,
@@ -113,10 +115,9 @@ __global__ __launch_bounds__(256) void conv_wgrad_mfma(const uint16_t* __restric
                                                        int H, int W) {
   constexpr int K = 27 * CIN;
   constexpr int NB = (COUT + 15) / 16;
-  __shared__ float red[4][NB][4][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m = lane & 15, g = lane >> 4;
-  const int kk = blockIdx.y * 16 + m;  // B column of this lane
+  const int kk = blockIdx.y * 64 + wave * 16 + m;  // B column of this lane
   const bool kvalid = kk < K;
   const int tap = kvalid ? kk / CIN : 0, ci = kvalid ? kk - (kk / CIN) * CIN : 0;
   const int dz = tap / 9, dyy = (tap / 3) % 3, dx = tap % 3;
@@ -124,7 +125,7 @@ __global__ __launch_bounds__(256) void conv_wgrad_mfma(const uint16_t* __restric
   f32x4 acc[NB];
 #pragma unroll
   for (int cb = 0; cb < NB; ++cb) acc[cb] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int rr = wave; rr < kRows; rr += 4) {
+  for (int rr = 0; rr < kRows; ++rr) {
     const int row = blockIdx.x * kRows + rr;
     if (row >= nrows) break;
     const int yv = row % H, nz = row / H;
@@ -150,20 +151,12 @@ __global__ __launch_bounds__(256) void conv_wgrad_mfma(const uint16_t* __restric
 #pragma unroll
   for (int cb = 0; cb < NB; ++cb)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) red[wave][cb][r][lane] = acc[cb][r];
-  __syncthreads();
-  if (wave == 0) {
-#pragma unroll
-    for (int cb = 0; cb < NB; ++cb)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const float v = red[0][cb][r][lane] + red[1][cb][r][lane] + red[2][cb][r][lane] +
-                        red[3][cb][r][lane];
-        const int co = cb * 16 + g * 4 + r;  // D row
-        const int kc = blockIdx.y * 16 + m; // D column
-        if (co < COUT && kc < K && v != 0.0f) unsafeAtomicAdd(dw + (size_t)co * K + kc, v);
-      }
-  }
+    for (int r = 0; r < 4; ++r) {
+      const float v = acc[cb][r];
+      const int co = cb * 16 + g * 4 + r;                  // D row
+      const int kc = blockIdx.y * 64 + wave * 16 + m;      // D column
+      if (co < COUT && kc < K && v != 0.0f) unsafeAtomicAdd(dw + (size_t)co * K + kc, v);
+    }
 }
 
 // ---- 1-channel layers (FP32 VALU) ------------------------------------------------------------
@@ -378,7 +371,7 @@ bool launch_wgrad(int cin, int cout, const void* x, const void* dy, float* dw, i
                   int H, int W, hipStream_t s) {
   const int K = 27 * cin;
   const int rows = N * D * H;
-  dim3 grid((rows + kRows - 1) / kRows, (K + 15) / 16);
+  dim3 grid((rows + kRows - 1) / kRows, (K + 63) / 64);
 #define FC_WG_CASE(ci, co)                                                                   \
   if (cin == ci && cout == co) {                                                             \
     conv_wgrad_mfma<ci, co><<<grid, 256, 0, s>>>((const uint16_t*)x, (const uint16_t*)dy, dw, \
