@@ -29,6 +29,7 @@ namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
@@ -134,17 +135,42 @@ __global__ __launch_bounds__(256) void conv_wgrad_mfma(const uint16_t* __restric
     const bool rvalid = kvalid && zz >= 0 && zz < D && yy >= 0 && yy < H;
     const size_t xrow = (((size_t)n * D + zz) * H + yy) * W;
     const size_t yrow = (((size_t)n * D + z) * H + yv) * W;
-    for (int x0 = 0; x0 < W; x0 += 4) {
-      const int xv = x0 + g;  // voxel of this lane's K slot
-      const int xx = xv + dx - 1;
-      float bval = 0.0f;
-      if (rvalid && xv < W && xx >= 0 && xx < W) bval = bf2f(x[(xrow + xx) * CIN + ci]);
+    // 32 voxels per step: lane (m, g) supplies voxels x0 + 8 g + q, q = 0..7, of its output
+    // channel (A) and of its K column (B) as raw BF16 bits
+    for (int x0 = 0; x0 < W; x0 += 32) {
+      uint32_t bw[4];
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        uint32_t pr = 0u;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const int xv = x0 + g * 8 + 2 * h + e;
+          const int xx = xv + dx - 1;
+          uint32_t v = 0u;
+          if (rvalid && xv < W && xx >= 0 && xx < W) v = x[(xrow + xx) * CIN + ci];
+          pr |= v << (16 * e);
+        }
+        bw[h] = pr;
+      }
+      const bf16x8 bv = __builtin_bit_cast(bf16x8, u32x4{bw[0], bw[1], bw[2], bw[3]});
 #pragma unroll
       for (int cb = 0; cb < NB; ++cb) {
         const int co = cb * 16 + m;
-        float aval = 0.0f;
-        if (co < COUT && xv < W) aval = bf2f(dy[(yrow + xv) * COUT + co]);
-        acc[cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(aval, bval, acc[cb], 0, 0, 0);
+        uint32_t aw[4];
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+          uint32_t pr = 0u;
+#pragma unroll
+          for (int e = 0; e < 2; ++e) {
+            const int xv = x0 + g * 8 + 2 * h + e;
+            uint32_t v = 0u;
+            if (co < COUT && xv < W) v = dy[(yrow + xv) * COUT + co];
+            pr |= v << (16 * e);
+          }
+          aw[h] = pr;
+        }
+        const bf16x8 av = __builtin_bit_cast(bf16x8, u32x4{aw[0], aw[1], aw[2], aw[3]});
+        acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[cb], 0, 0, 0);
       }
     }
   }
