@@ -109,6 +109,8 @@ __global__ __launch_bounds__(256) void flip_transpose_weights(const uint16_t* __
 // The four waves read the same output-gradient values (served by L1 after the first wave), so each
 // is fetched from L2 once per 64 columns of K instead of once per 16. dW (FP32) is accumulated.
 constexpr int kRows = 8;
+constexpr int kWgChunk = 256;            // voxels of a row staged at once
+constexpr int kWgPitch = kWgChunk + 8;    // LDS row pitch (elements): 16-byte aligned rows
 template <int CIN, int COUT>
 __global__ __launch_bounds__(256) void conv_wgrad_mfma(const uint16_t* __restrict__ x,
                                                        const uint16_t* __restrict__ dy,
@@ -116,6 +118,8 @@ __global__ __launch_bounds__(256) void conv_wgrad_mfma(const uint16_t* __restric
                                                        int H, int W) {
   constexpr int K = 27 * CIN;
   constexpr int NB = (COUT + 15) / 16;
+  // the output-gradient chunk of the current row, transposed: T[co][voxel - chunk start]
+  __shared__ __align__(16) uint16_t T[NB * 16][kWgPitch];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m = lane & 15, g = lane >> 4;
   const int kk = blockIdx.y * 64 + wave * 16 + m;  // B column of this lane
@@ -128,49 +132,53 @@ __global__ __launch_bounds__(256) void conv_wgrad_mfma(const uint16_t* __restric
   for (int cb = 0; cb < NB; ++cb) acc[cb] = f32x4{0.f, 0.f, 0.f, 0.f};
   for (int rr = 0; rr < kRows; ++rr) {
     const int row = blockIdx.x * kRows + rr;
-    if (row >= nrows) break;
+    if (row >= nrows) break;  // uniform over the block
     const int yv = row % H, nz = row / H;
     const int z = nz % D, n = nz / D;
     const int zz = z + dz - 1, yy = yv + dyy - 1;
     const bool rvalid = kvalid && zz >= 0 && zz < D && yy >= 0 && yy < H;
     const size_t xrow = (((size_t)n * D + zz) * H + yy) * W;
     const size_t yrow = (((size_t)n * D + z) * H + yv) * W;
-    // 32 voxels per step: lane (m, g) supplies voxels x0 + 8 g + q, q = 0..7, of its output
-    // channel (A) and of its K column (B) as raw BF16 bits
-    for (int x0 = 0; x0 < W; x0 += 32) {
-      uint32_t bw[4];
+    for (int xc = 0; xc < W; xc += kWgChunk) {
+      const int nv = min(kWgChunk, W - xc);
+      const int nvp = (nv + 31) & ~31;  // the x0 loop reads whole 32-voxel steps
+      __syncthreads();  // the previous chunk is no longer read
+      // stage: 8 consecutive channels of one voxel per 16-byte load, zero past the row's end
+      for (int e = threadIdx.x; e < nvp * (COUT / 8); e += 256) {
+        const int v = e / (COUT / 8), c8 = (e - v * (COUT / 8)) * 8;
+        u32x4 q = u32x4{0u, 0u, 0u, 0u};
+        if (v < nv) q = *reinterpret_cast<const u32x4*>(dy + (yrow + xc + v) * COUT + c8);
 #pragma unroll
-      for (int h = 0; h < 4; ++h) {
-        uint32_t pr = 0u;
-#pragma unroll
-        for (int e = 0; e < 2; ++e) {
-          const int xv = x0 + g * 8 + 2 * h + e;
-          const int xx = xv + dx - 1;
-          uint32_t v = 0u;
-          if (rvalid && xv < W && xx >= 0 && xx < W) v = x[(xrow + xx) * CIN + ci];
-          pr |= v << (16 * e);
+        for (int h = 0; h < 4; ++h) {
+          T[c8 + 2 * h][v] = (uint16_t)(q[h] & 0xffffu);
+          T[c8 + 2 * h + 1][v] = (uint16_t)(q[h] >> 16);
         }
-        bw[h] = pr;
       }
-      const bf16x8 bv = __builtin_bit_cast(bf16x8, u32x4{bw[0], bw[1], bw[2], bw[3]});
-#pragma unroll
-      for (int cb = 0; cb < NB; ++cb) {
-        const int co = cb * 16 + m;
-        uint32_t aw[4];
+      __syncthreads();
+      // 32 voxels per step: lane (m, g) supplies voxels x0 + 8 g + q, q = 0..7, of its output
+      // channel (A, one 16-byte LDS read) and of its K column (B, gathered) as raw BF16 bits
+      for (int x0 = 0; x0 < nv; x0 += 32) {
+        uint32_t bw[4];
 #pragma unroll
         for (int h = 0; h < 4; ++h) {
           uint32_t pr = 0u;
 #pragma unroll
           for (int e = 0; e < 2; ++e) {
-            const int xv = x0 + g * 8 + 2 * h + e;
+            const int xv = xc + x0 + g * 8 + 2 * h + e;
+            const int xx = xv + dx - 1;
             uint32_t v = 0u;
-            if (co < COUT && xv < W) v = dy[(yrow + xv) * COUT + co];
+            if (rvalid && xv < W && xx >= 0 && xx < W) v = x[(xrow + xx) * CIN + ci];
             pr |= v << (16 * e);
           }
-          aw[h] = pr;
+          bw[h] = pr;
         }
-        const bf16x8 av = __builtin_bit_cast(bf16x8, u32x4{aw[0], aw[1], aw[2], aw[3]});
-        acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[cb], 0, 0, 0);
+        const bf16x8 bv = __builtin_bit_cast(bf16x8, u32x4{bw[0], bw[1], bw[2], bw[3]});
+#pragma unroll
+        for (int cb = 0; cb < NB; ++cb) {
+          const u32x4 aq = *reinterpret_cast<const u32x4*>(&T[cb * 16 + m][x0 + g * 8]);
+          acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, aq), bv,
+                                                            acc[cb], 0, 0, 0);
+        }
       }
     }
   }

@@ -103,3 +103,27 @@ def test_unet_matches_fp32_reference(gpu):
     corr = out.detach().cpu() - vol.cpu()
     corr_ref = ref - vol.cpu()
     assert rel(corr, corr_ref) < 5e-2
+
+
+@pytest.mark.parametrize("cin,cout", [(16, 16), (32, 64)])
+def test_weight_gradient_long_rows(gpu, cin, cout):
+    # rows longer than the 256 voxels the weight-gradient kernel stages at once (300 = 256 + 44)
+    from fastcone_amd import _native
+    g = torch.Generator().manual_seed(7 + cin + cout)
+    N, D, H, W = 1, 3, 4, 300
+    x = torch.randn((N, D, H, W, cin), generator=g).to(torch.bfloat16)
+    gy = torch.randn((N, D, H, W, cout), generator=g).to(torch.bfloat16)
+    dw = torch.zeros((cout, 27, cin), dtype=torch.float32, device=gpu)
+    xg, gyg = x.to(gpu).contiguous(), gy.to(gpu).contiguous()
+    L = _native.lib()
+    s = torch.cuda.current_stream(gpu).cuda_stream
+    import ctypes
+    _native.check(L.fc_conv3d_wgrad_dev(gpu.index, xg.data_ptr(), gyg.data_ptr(), dw.data_ptr(),
+                                        N, D, H, W, cin, cout, ctypes.c_void_p(int(s))))
+    torch.cuda.synchronize()
+    # reference: d/dw of sum(conv3d(x, w) * gy) in FP32 on the CPU
+    w0 = torch.zeros((cout, cin, 3, 3, 3), requires_grad=True)
+    y = torch.nn.functional.conv3d(x.float().permute(0, 4, 1, 2, 3), w0, padding=1)
+    (y * gy.float().permute(0, 4, 1, 2, 3)).sum().backward()
+    ref = w0.grad.permute(0, 2, 3, 4, 1).reshape(cout, 27, cin)
+    assert rel(dw.cpu(), ref) < 2e-3
