@@ -472,3 +472,38 @@ def test_abi_calls_with_no_views(gpu):
     cone.backproject(v, proj, np.zeros(0, np.float32), 70.0, np.zeros(0, np.float32),
                      norm=np.zeros(0, np.float32))
     assert np.all(v == 0)
+
+
+@pytest.mark.parametrize("quad", [None, "always", "never"])
+def test_project_random_dense_geometries(gpu, quad, monkeypatch):
+    # seeded random geometries with adjacent views 0.1 - 2 deg apart (the quad layouts' domain):
+    # windows, block offsets, steps, source distances down to 20 voxels and, in 40 % of the
+    # cases, a tilted, shifted detector
+    from fastcone_amd import ops
+    from fastcone_amd.geometry import Geometry
+    if quad is not None:
+        monkeypatch.setenv("FASTCONE_FP_QUAD", quad)
+    rng = np.random.default_rng(2024)
+    for case in range(30):
+        nv = int(rng.integers(1, 21))
+        ang = rng.uniform(0, 2 * np.pi) + np.deg2rad(rng.uniform(0.1, 2.0)) * np.arange(nv)
+        D, pw = float(rng.uniform(20.0, 150.0)), float(rng.uniform(0.5, 1.5))
+        vol = tuple(int(v) for v in rng.integers(6, 40, 3))
+        vol_dim = tuple(v + int(rng.integers(0, 4)) for v in vol)
+        vs = tuple(int(rng.integers(0, d - v + 1)) for v, d in zip(vol, vol_dim))
+        det = (int(rng.integers(6, 70)), int(rng.integers(6, 70)))
+        det_dim = (det[0] + int(rng.integers(0, 6)), det[1] + int(rng.integers(0, 6)))
+        hmid = rng.uniform(vol_dim[0] / 2 - 2, vol_dim[0] / 2 + 2, nv)
+        woff, hoff = rng.integers(-3, 5, nv), rng.integers(-3, 5, nv)
+        mis = None
+        if rng.uniform() < 0.4:
+            mis = list(rng.uniform(-1.0, 1.0, 3)) + list(rng.uniform(-0.3, 0.3, 3))
+        h = float(rng.choice([0.37, 0.5, 1.0]))
+        g = Geometry.circular(ang, D, pw, vol, det, hmid=hmid, vol_dim=vol_dim, vol_start=vs,
+                              woff=woff, hoff=hoff, misalign=mis, det_dim=det_dim)
+        prob = oracle.Problem(ang, hmid, D, pw, vol_dim=vol_dim[::-1], vol_start=vs[::-1],
+                              proj_dim=det_dim[::-1], woff=woff, hoff=hoff, mis=mis)
+        x = rng.standard_normal(vol).astype(np.float32)
+        ref = oracle.project(prob, x, window=det[::-1], h=h)
+        p = ops.project(torch.from_numpy(x).to(gpu), g, step=h).cpu()
+        assert rel(p, ref) < 5e-5, (case, nv, D, pw, vol, det, mis, h)
