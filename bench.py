@@ -6,7 +6,7 @@ centres within 60 mm) plus N(0, 0.001^2)/mm texture; 720 views over 360 deg; fla
 768 x 768, pitch 0.75 voxel at the axis; source-to-axis 1200 voxels (600 mm); ray step 0.5 voxel.
 
 One step = forward projection (stage + ray-march kernel) -> FDK filtering (cosine weight +
-ramp via torch.fft) -> FDK backprojection (stage + voxel-driven kernel), all device-resident.
+ramp, one FFT kernel) -> FDK backprojection (stage + voxel-driven kernel), all device-resident.
 value = W / step time, W = 512^3 * 720 voxel-view updates (Gvoxel-view/s).
 
 Extra fields: per-kernel rates, the gather roofline of the dominant kernel (the projector), the
@@ -89,7 +89,6 @@ def main():
         vol, _ = phantom.config_phantom(1, device=dev)
     ang = 2 * np.pi * np.arange(nv) / nv
     geom = Geometry.circular(ang, D, PW, (N, N, N), (NDET, NDET))
-    weights = ops.fdk_cosine_weights(geom, D, PW, device=dev)
     norm = torch.full((nv,), 1.0 / nv, dtype=torch.float32, device=dev)
     proj = torch.zeros(geom.proj_shape, dtype=torch.float32, device=dev)
     rec = torch.zeros(geom.vol_shape, dtype=torch.float32, device=dev)
@@ -100,7 +99,7 @@ def main():
     def step(v, store=torch.float32):
         proj.zero_()
         ops.project(v, geom, step=H, out=proj)
-        q = ops.fdk_filter(proj, geom, D, PW, out_dtype=store, weights=weights)
+        q = ops.fdk_filter(proj, geom, D, PW, out_dtype=store)
         rec.zero_()
         ops.backproject(q, geom, _native.FC_BP_FDK, out=rec, norm=norm)
 

@@ -14,6 +14,7 @@ FC_OK, FC_ERR_ARGUMENT, FC_ERR_DEVICE, FC_ERR_HIP, FC_ERR_MEMORY = 0, 1, 2, 3, 4
 FC_F32, FC_BF16, FC_F16 = 0, 1, 2
 FC_BP_UNWEIGHTED, FC_BP_KATSEVICH, FC_BP_FDK = 0, 1, 2
 FC_DET_FLAT, FC_DET_CYLINDER = 0, 1
+FC_FILTER_MAX_COLS = 4096
 FC_VIEW_DOUBLES = 32
 
 _lib = None
@@ -49,6 +50,7 @@ def _declare(lib):
                                    _vp]),
         "fc_sample_count_dev": (i, [i, _c_int_p, _c_dbl_p, _c_int_p, _vp, d, _vp, _vp]),
         "fc_convert_dev": (i, [i, _vp, i, _vp, i, ll, _vp]),
+        "fc_fdk_filter_dev": (i, [i, _vp, i, _vp, i, _c_int_p, _vp, d, _vp]),
         "fc_conv3d_fwd_dev": (i, [i, _vp, _vp, _vp, _vp, i, i, i, i, i, i, i, _vp]),
         "fc_conv3d_dgrad_dev": (i, [i, _vp, _vp, _vp, _vp, i, i, i, i, i, i, _vp]),
         "fc_conv3d_wgrad_dev": (i, [i, _vp, _vp, _vp, i, i, i, i, i, i, _vp]),

@@ -19,6 +19,7 @@ LIB = os.path.join(HERE, "libFastcone.so")
 ARCH = os.environ.get("FASTCONE_ARCH", "gfx950")
 
 HIP_SOURCES = ["fc_runtime.hip", "fc_project.hip", "fc_backproject.hip", "fc_util.hip",
+               "fc_filter.hip",
                "fc_geometry.cpp", "fc_katsevich.hip", "fc_conv3d.hip"]
 C_SOURCES = ["wrapper.c"]
 

@@ -51,12 +51,18 @@ class CGLSState:
                    int(z["it"]), None if np.isnan(atb) else atb)
 
 
+def _finite(v, what):
+    if not bool(torch.isfinite(v)):
+        raise FloatingPointError(f"CGLS: {what} is not finite (non-finite data or start point)")
+
+
 def cgls(A, At, b, x0=None, iters=50, tol=None, atb_norm=None, state=None, callback=None,
          sync_every=1):
     """Run CGLS. A(x) -> projections, At(r) -> volume (exact transpose of A).
 
     Returns (x, info) with info = {"residual": [|A^T r_k| / |A^T b|], "rnorm": [|r_k|],
-    "iters", "time_s", "converged", "state"}. `state` resumes a checkpointed run.
+    "iters", "time_s", "converged", "state"}. `state` resumes a checkpointed run. Raises
+    FloatingPointError when an inner product turns non-finite (NaN or infinite data).
     """
     t0 = time.perf_counter()
     if state is None:
@@ -70,6 +76,7 @@ def cgls(A, At, b, x0=None, iters=50, tol=None, atb_norm=None, state=None, callb
             s = At(r)
         p = s.clone()
         gamma = dot64(s, s)
+        _finite(gamma, "|A^T r|^2")
         if atb_norm is None:
             atb_norm = float(torch.sqrt(gamma)) if x0 is None else float(norm64(At(b)))
         it0 = 0
@@ -83,11 +90,13 @@ def cgls(A, At, b, x0=None, iters=50, tol=None, atb_norm=None, state=None, callb
     while not converged and it < iters:
         q = A(p)
         qq = dot64(q, q)
+        _finite(qq, "|A p|^2")
         alpha = (gamma / qq).float()
         x.add_(alpha * p)
         r.sub_(alpha * q)
         s = At(r)
         gamma_new = dot64(s, s)
+        _finite(gamma_new, "|A^T r|^2")
         beta = (gamma_new / gamma).float()
         p.mul_(beta).add_(s)
         gamma = gamma_new

@@ -45,7 +45,7 @@ int select_device(int device);
 // makes the new stream wait on an event recorded after that previous use, so reuse is ordered.
 enum Slot { SLOT_OCTET = 0, SLOT_QUAD = 1, SLOT_H2D_VOL = 2, SLOT_H2D_PROJ = 3,
             SLOT_H2D_VIEWS = 4, SLOT_H2D_NORM = 5, SLOT_KAT_A = 6, SLOT_KAT_B = 7,
-            SLOT_KAT_TAB = 8, SLOT_COUNT = 9 };
+            SLOT_KAT_TAB = 8, SLOT_FILTER = 9, SLOT_COUNT = 10 };
 
 struct Workspace {
   void* ptr = nullptr;
@@ -89,6 +89,14 @@ int launch_stage_quads(const void* proj, int dt, int nw, int nh, int nv, const f
                        void* quad, hipStream_t stream);
 int launch_gather_bench_fp(int device, int dt, int plain, double* samples_per_s);
 int launch_gather_bench_bp(int device, int dt, int ray, double* samples_per_s);
+// Row filter (fc_filter.hip): out[r, j] = sum_m kern[j - m] wgt(r, m) in[r, m] over nrows rows
+// of nw values (nh rows per view for the cosine weight, views may be null = no weight);
+// kind fc_filter_kind with parameter p0 (ramp scale, or the Hilbert kernel's column pitch).
+// `tables` is a device buffer of filter_table_bytes(nw) bytes. in and out may alias.
+size_t filter_table_bytes(int nw);
+int launch_row_filter(const void* in, int in_dt, void* out, int out_dt, int nw, int nh,
+                      long long nrows, const double* views, int kind, double p0, void* tables,
+                      hipStream_t stream);
 int launch_convert(const void* src, int src_dtype, void* dst, int dst_dtype, long long n,
                    hipStream_t stream);
 

@@ -20,8 +20,8 @@
 //
 // Layout: every stage is a dense [view][row][column] FP32 array in HBM (c2: 0.57 GB per stage,
 // 1.1 GB for the kappa-line stages); the backprojector reads g5 through the quad layout of
-// fc_backproject.hip. The Hilbert stage is a direct LDS convolution per kappa line (odd taps
-// only), the other filter stages are elementwise gathers.
+// fc_backproject.hip. The Hilbert stage is the FFT row filter of fc_filter.hip (one convolution
+// per kappa line), the other filter stages are elementwise gathers.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -123,37 +123,9 @@ __global__ __launch_bounds__(256) void kat_forward_rebin(const float* __restrict
   g3[((size_t)k * g.npsi + m) * g.nw + j] = s;
 }
 
-// Hilbert transform of one kappa line per block: g4[j] = sum_{n odd} h[n] g3[j - n],
-// h[n] = 2/(pi n) * n dg / sin(n dg). The line and the kernel sit in LDS.
-constexpr int kMaxCols = 2048;
-__global__ __launch_bounds__(256) void kat_hilbert(const float* __restrict__ g3, KatGeom g,
-                                                   const float* __restrict__ hk,
-                                                   float* __restrict__ g4) {
-  __shared__ float line[kMaxCols];
-  __shared__ float ker[2 * kMaxCols];
-  const size_t base = ((size_t)blockIdx.z * g.npsi + blockIdx.y) * g.nw;
-  for (int j = threadIdx.x; j < g.nw; j += 256) line[j] = g3[base + j];
-  for (int n = threadIdx.x; n < 2 * g.nw - 1; n += 256) ker[n] = hk[n];
-  __syncthreads();
-  for (int j = threadIdx.x; j < g.nw; j += 256) {
-    // odd n = j - jj  <=>  jj of opposite parity to j
-    float acc = 0.0f;
-    for (int jj = (j & 1) ^ 1; jj < g.nw; jj += 2) acc = fmaf(ker[j - jj + g.nw - 1], line[jj], acc);
-    g4[base + j] = acc;
-  }
-}
-
-__global__ __launch_bounds__(256) void kat_hilbert_kernel(KatGeom g, float* __restrict__ hk) {
-  int q = blockIdx.x * 256 + threadIdx.x;
-  if (q >= 2 * g.nw - 1) return;
-  int n = q - (g.nw - 1);
-  float v = 0.0f;
-  if (n & 1) {
-    double x = n * g.dg;
-    v = (float)(2.0 / (M_PI * n) * (x / sin(x)));
-  }
-  hk[q] = v;
-}
+// The Hilbert transform of each kappa line, g4[j] = sum_{n odd} h[n] g3[j - n] with
+// h[n] = 2/(pi n) * n dg / sin(n dg), is the FFT row filter's FC_FILTER_HILBERT kernel.
+constexpr int kMaxCols = 2048;  // detector columns (FFT length <= 4096 in the Hilbert stage)
 
 // g5[k][i][j] = g4 interpolated between kappa lines bracketing row i at column j.
 __global__ __launch_bounds__(256) void kat_backward_rebin(const float* __restrict__ g4, KatGeom g,
@@ -307,17 +279,17 @@ extern "C" int fc_katsevich_dev(int device, const void* proj, int proj_dtype, in
   const size_t big = (size_t)nviews * (plane > kplane ? plane : kplane) * sizeof(float);
   if ((rc = acquire_ws(device, SLOT_KAT_A, big, stream, &A))) return rc;
   if ((rc = acquire_ws(device, SLOT_KAT_B, big, stream, &B))) return rc;
-  const size_t tab = (kplane + plane + 2 * (size_t)nw + 4 * (size_t)nviews + 4) * sizeof(float);
+  const size_t tab = (kplane + plane + 4 * (size_t)nviews + 4) * sizeof(float);
   if ((rc = acquire_ws(device, SLOT_KAT_TAB, tab, stream, &T))) return rc;
+  void* ftab = nullptr;
+  if ((rc = acquire_ws(device, SLOT_FILTER, filter_table_bytes(nw), stream, &ftab))) return rc;
   float* fwd = static_cast<float*>(T);
   float* bwd = fwd + kplane;
-  float* hk = bwd + plane;
-  float4* vtab = reinterpret_cast<float4*>(fwd + ((kplane + plane + 2 * (size_t)nw + 3) & ~(size_t)3));
+  float4* vtab = reinterpret_cast<float4*>(fwd + ((kplane + plane + 3) & ~(size_t)3));
 
   int tok = prof_begin("katsevich_filter", stream);
   const int bx = (nw + 255) / 256;
   kat_tables<<<bx, 256, 0, stream>>>(g, fwd, bwd);
-  kat_hilbert_kernel<<<(2 * nw - 1 + 255) / 256, 256, 0, stream>>>(g, hk);
   float* g2 = static_cast<float*>(A);
   if (proj_dtype == FC_F32) kat_derivative<FC_F32><<<dim3(bx, nh, nviews), 256, 0, stream>>>(proj, g, g2);
   else if (proj_dtype == FC_BF16) kat_derivative<FC_BF16><<<dim3(bx, nh, nviews), 256, 0, stream>>>(proj, g, g2);
@@ -325,7 +297,10 @@ extern "C" int fc_katsevich_dev(int device, const void* proj, int proj_dtype, in
   float* g3 = static_cast<float*>(B);
   kat_forward_rebin<<<dim3(bx, npsi, nviews), 256, 0, stream>>>(g2, g, fwd, g3);
   float* g4 = static_cast<float*>(A);  // g2 is no longer needed
-  kat_hilbert<<<dim3(1, npsi, nviews), 256, 0, stream>>>(g3, g, hk, g4);
+  // Hilbert transform along every kappa line: FFT convolution with the curved-detector kernel
+  if ((rc = launch_row_filter(g3, FC_F32, g4, FC_F32, nw, npsi, (long long)npsi * nviews, nullptr,
+                              FC_FILTER_HILBERT, dgamma, ftab, stream)))
+    return rc;
   float* g5 = static_cast<float*>(B);  // g3 is no longer needed
   kat_backward_rebin<<<dim3(bx, nh, nviews), 256, 0, stream>>>(g4, g, bwd, g5);
   prof_end(tok, stream);
@@ -358,5 +333,6 @@ extern "C" int fc_katsevich_dev(int device, const void* proj, int proj_dtype, in
   if ((rc = release_ws(device, SLOT_QUAD, stream))) return rc;
   if ((rc = release_ws(device, SLOT_KAT_A, stream))) return rc;
   if ((rc = release_ws(device, SLOT_KAT_B, stream))) return rc;
+  if ((rc = release_ws(device, SLOT_FILTER, stream))) return rc;
   return release_ws(device, SLOT_KAT_TAB, stream);
 }

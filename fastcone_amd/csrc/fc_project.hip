@@ -665,7 +665,7 @@ __device__ __forceinline__ void scatter_global(float* accp, uint32_t PX, uint32_
 // corner floor(q) lies in the brick — so every sample of every ray is scattered by exactly one
 // block — and adds y h norm times its eight trilinear weights into a 17^3 LDS accumulator. The
 // accumulator is 64-bit fixed point: each contribution is scaled by 2^28 / max|y h norm| (a device
-// reduction over the projections, so |contribution| < 2^28), truncated to an integer and added with
+// reduction over the projections, so |contribution| < 2^28), rounded to an integer and added with
 // ds_add_u64, which runs about 30x faster than an LDS float atomic on this GPU; the resolution is
 // 2^-28 of the largest datum and the 64-bit sums cannot overflow. The accumulator is converted
 // back and added to the volume once at the end (one global float atomic per nonzero element;
@@ -678,7 +678,7 @@ constexpr int kAT = 512;           // threads per brick (8 waves share one accum
 
 struct AdjVArgs {
   float* vol;          // unpadded output block, accumulated
-  unsigned int* ymax;  // bits of max |y h norm| over the stack (non-negative float)
+  unsigned int* ymax;  // [0]: bits of max |y h norm| over the stack; [1]: non-finite data flag
   float4* rays;        // per ray: (px, py, pz, sx), (sy, sz, nk bits, y h norm); see adj_rays
   const void* proj;
   const double* views;
@@ -717,8 +717,13 @@ __global__ __launch_bounds__(256) void adj_rays(AdjVArgs a) {
       r = ray_segment(a.views + (size_t)t * FC_VIEW_DOUBLES, j, i, a.ox, a.oy, a.oz, a.nx, a.ny,
                       a.nz, a.h, a.max_steps, 2.0f);
     a.rays[2 * idx] = make_float4(r.px, r.py, r.pz, r.sx);
-    a.rays[2 * idx + 1] = make_float4(r.sy, r.sz, __int_as_float(r.nk), r.nk > 0 ? y : 0.0f);
-    m = r.nk > 0 ? fabsf(y) : 0.0f;
+    a.rays[2 * idx + 1] =
+        make_float4(r.sy, r.sz, __int_as_float(r.nk), r.nk > 0 && fabsf(y) <= 3.402823466e38f ? y : 0.0f);
+    // a non-finite datum on a ray that reaches the block cannot be held by the fixed-point
+    // accumulator: it is flagged (the whole output becomes NaN) and kept out of the maximum
+    const bool bad = r.nk > 0 && !(fabsf(y) <= 3.402823466e38f);
+    if (bad) atomicOr(a.ymax + 1, 1u);
+    m = r.nk > 0 && !bad ? fabsf(y) : 0.0f;
   }
   red[threadIdx.x] = m;
   __syncthreads();
@@ -774,13 +779,26 @@ __global__ __launch_bounds__(256) void adj_windows(AdjVArgs a, uint2* __restrict
 __global__ __launch_bounds__(kAT) void adjoint_vox_kernel(AdjVArgs a) {
   __shared__ unsigned long long acc[kAL3];
   for (int e = threadIdx.x; e < kAL3; e += kAT) acc[e] = 0ull;
-  const float ymax = __uint_as_float(*a.ymax);
-  if (!(ymax > 0.0f) || !(ymax < 3.0e38f)) return;  // all-zero data (uniform exit)
-  const float fscale = 268435456.0f / ymax;         // 2^28 / max|y|
-  __syncthreads();
   const int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z % a.nbz;
   // this block's share of the views (small volumes split the views over several blocks)
   const int split = blockIdx.z / a.nbz;
+  if (a.ymax[1] != 0u) {
+    // non-finite data reach the volume: the transpose is NaN (uniform exit). Each voxel
+    // belongs to the base-cell range [16 b - 1, 16 b + 15) of exactly one brick.
+    if (split == 0) {
+      for (int e = threadIdx.x; e < kAB * kAB * kAB; e += kAT) {
+        const int x = kAB * bx - 1 + e % kAB, yy = kAB * by - 1 + (e / kAB) % kAB,
+                  z = kAB * bz - 1 + e / (kAB * kAB);
+        if (x >= 0 && yy >= 0 && z >= 0 && x < a.nx && yy < a.ny && z < a.nz)
+          a.vol[((size_t)z * a.ny + yy) * a.nx + x] = __builtin_nanf("");
+      }
+    }
+    return;
+  }
+  const float ymax = __uint_as_float(a.ymax[0]);
+  if (!(ymax > 0.0f)) return;                // all-zero data (uniform exit)
+  const float fscale = 268435456.0f / ymax;  // 2^28 / max|y|
+  __syncthreads();
   const int tbeg = (int)((long long)a.nviews * split / a.nsplit);
   const int tend = (int)((long long)a.nviews * (split + 1) / a.nsplit);
   // base-cell range in the padded sample frame (voxel index + 2)
@@ -842,9 +860,9 @@ __global__ __launch_bounds__(kAT) void adjoint_vox_kernel(AdjVArgs a) {
     int run = 0;
     auto flush = [&]() {
       unsigned long long* L = acc + cur;
-      // truncate to an integer, sign-extend, add in 64-bit two's complement
+      // round to the nearest integer, sign-extend, add in 64-bit two's complement
       auto add = [](unsigned long long* p, float v) {
-        atomicAdd(p, (unsigned long long)(long long)(int)v);
+        atomicAdd(p, (unsigned long long)(long long)__float2int_rn(v));
       };
       add(L, A[0]); add(L + 1, A[1]); add(L + kAL, A[2]); add(L + kAL + 1, A[3]);
       add(L + kAL * kAL, A[4]); add(L + kAL * kAL + 1, A[5]);
@@ -1278,7 +1296,7 @@ int launch_adjoint(int device, float* vol, const int vs[3], const double origin[
   a.win = win;
   a.ymax = reinterpret_cast<unsigned int*>(static_cast<char*>(ws) + moff);
   int tok = prof_begin("adjoint", stream);
-  zero_fill<unsigned int><<<1, 256, 0, stream>>>(a.ymax, 1);
+  zero_fill<unsigned int><<<1, 256, 0, stream>>>(a.ymax, 2);  // max |y| bits, non-finite flag
   const unsigned rblocks = (unsigned)((nrays + 255) / 256);
   if (dt == FC_F32) adj_rays<FC_F32><<<rblocks, 256, 0, stream>>>(a);
   else if (dt == FC_BF16) adj_rays<FC_BF16><<<rblocks, 256, 0, stream>>>(a);

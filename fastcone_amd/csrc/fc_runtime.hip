@@ -386,15 +386,10 @@ static int cb_validate(float* volume, float* projections, float* angles, float* 
   FC_CHECK_ARG(std::isfinite(pw) && pw > 0.0f, "pixelWidth must be > 0");
   FC_CHECK_ARG(volume && projections, "null data pointer");
   if (projDim[2] > 0) FC_CHECK_ARG(angles && hMid && woff && hoff && norm, "null per-view array");
-  // Every voxel of the block must lie strictly in front of the source plane for all views, so the
-  // flat-detector projection is defined (D larger than the block's farthest corner radius).
-  double r2 = 0.0;
-  for (int a = 0; a < 2; ++a) {
-    double lo = (double)volStart[a] - volDim[a] / 2 - 1.0, hi = (double)volEnd[a] - volDim[a] / 2;
-    double m = std::max(std::fabs(lo), std::fabs(hi));
-    r2 += m * m;
-  }
-  FC_CHECK_ARG(std::sqrt(r2) < 0.999 * D, "volume block reaches the source (D=%g)", (double)D);
+  // A block may reach past the source radius, as in the reference (its sweep start may be
+  // negative, coneBeamGPU.cu:1781): the projector clips every ray to the block, and the
+  // backprojector evaluates voxels behind the source plane with the same projective formula as
+  // the reference (a voxel exactly on that plane gives an infinite weight there too).
   return FC_OK;
 }
 

@@ -5,14 +5,13 @@
 * ``backproject(proj, geom, mode)``   voxel-driven FDK / Katsevich / unweighted backprojection
 * ``ConeBeamProject``                 torch.autograd.Function: forward = project, backward =
                                       adjoint (checked by dot-product and finite-difference tests)
-* ``fdk(proj, geom, D, pw)``          cosine weight + ramp filter (torch.fft) + FDK backprojection
-                                      in physical units (reconstructs the attenuation per voxel)
+* ``fdk_filter(proj, geom, D, pw)``   cosine weight + ramp filter (one HIP FFT kernel)
+* ``fdk(proj, geom, D, pw)``          fdk_filter + FDK backprojection in physical units
+                                      (reconstructs the attenuation per voxel)
 
 Tensors must live on a ROCm device; there is no CPU path. Volumes are [nz, ny, nx] and projection
 stacks [nviews, nh, nw], C-contiguous. Work is enqueued on torch's current stream.
 """
-import math
-
 import numpy as np
 import torch
 
@@ -197,70 +196,40 @@ def cone_adjoint(proj, geom, step=1.0):
 
 
 # ---- FDK ------------------------------------------------------------------------------------
-def ramp_response(nw, device, dtype=torch.float32):
-    """pi * k / (2 Nw) on the rfft bins of the 2 Nw zero-padded row, DC = pi/(16 Nw)
-    (the reference's filter, fastcone.py:278-281)."""
-    n = 2 * nw
-    H = math.pi * torch.arange(n // 2 + 1, dtype=torch.float64) / n
-    H[0] = math.pi * 0.125 / n
-    return H.to(device=device, dtype=dtype)
+def fdk_filter(proj, geom, D=None, pw=1.0, out_dtype=torch.float32, out=None, cosine=True):
+    """Cosine pre-weighting and ramp filtering of a projection stack on the device (one HIP kernel,
+    fc_fdk_filter_dev: FFT convolution of every detector row in LDS, FP32 arithmetic).
 
-
-def fdk_cosine_weights(geom, D, pw, device=None, chunk=32):
-    """FDK pre-weight of every window pixel: the cosine of the angle between its ray and the
-    detector normal, dperp / |P - src| (= D / sqrt(D^2 + u^2 + v^2) for an upright flat detector;
-    physical units, where the reference's fbp() uses pixel units). Flat-detector records.
-
-    Evaluated in FP64 on `device` (CPU by default), a chunk of views at a time, from six scalars per
-    view: |r0 + j du + i dv|^2 with r0 = d0 - src expands to a quadratic in (j, i). Returns an
-    FP32 tensor [nviews, nh, nw].
+    The ramp is the reference's (fastcone.py:272-292, cycles per pixel) divided by pw, so that
+    FDK returns the attenuation per voxel (the reference's fbp() returns pw times it); the cosine
+    weight is the cosine of the angle between each pixel's ray and the detector normal, from the
+    geometry's view records (physical units; the reference's fbp() uses pixel units). D is not
+    needed (the records carry the source) and is accepted for call compatibility. `out` may be
+    `proj` itself (in place) when the dtypes match.
     """
-    f64 = torch.float64
-    v = torch.as_tensor(np.ascontiguousarray(geom.views), dtype=f64, device=device)
-    nt = v.shape[0]
-    nh, nw = geom.det_shape
-    out = torch.empty((nt, nh, nw), dtype=torch.float32, device=device)
-    if nt == 0:
-        return out
-    src, d0, du, dv = v[:, 0:3], v[:, 3:6], v[:, 6:9], v[:, 9:12]
-    r0 = d0 - src
-    nrm = torch.linalg.cross(du, dv, dim=1)
-    nrm = nrm / torch.linalg.norm(nrm, dim=1, keepdim=True)
-    dperp = (r0 * nrm).sum(1).abs()
-    c0 = (r0 * r0).sum(1)
-    cj, ci = 2.0 * (r0 * du).sum(1), 2.0 * (r0 * dv).sum(1)
-    cjj, cii, cij = (du * du).sum(1), (dv * dv).sum(1), 2.0 * (du * dv).sum(1)
-    j = torch.arange(nw, dtype=f64, device=device)[None, None, :]
-    i = torch.arange(nh, dtype=f64, device=device)[None, :, None]
-    for t0 in range(0, nt, chunk):
-        sl = slice(t0, min(nt, t0 + chunk))
-        e = lambda c: c[sl][:, None, None]
-        dist2 = (e(c0) + e(cj) * j + e(ci) * i + e(cjj) * j * j + e(cii) * i * i
-                 + e(cij) * i * j)
-        out[sl] = (e(dperp) / torch.sqrt(dist2)).to(torch.float32)
+    _need_device(proj, "proj")
+    if tuple(proj.shape) != geom.proj_shape:
+        raise ValueError(f"proj shape {tuple(proj.shape)} != geometry {geom.proj_shape}")
+    if cosine and geom.kind != "flat":
+        raise ValueError("the FDK cosine weight needs a flat-detector geometry")
+    if out is None:
+        out = torch.empty(proj.shape, dtype=out_dtype, device=proj.device)
+    else:
+        _need_device(out, "out")
+        if tuple(out.shape) != tuple(proj.shape):
+            raise ValueError("out must have the projection shape")
+    views = geom.views_on(proj.device)
+    L = _native.lib()
+    rc = L.fc_fdk_filter_dev(proj.device.index, proj.data_ptr(), _dtype_code(proj, "proj"),
+                             out.data_ptr(), _dtype_code(out, "out"), _proj3(geom),
+                             views.data_ptr() if cosine else None, 1.0 / float(pw),
+                             _stream(proj.device))
+    _native.check(rc)
     return out
 
 
-def fdk_filter(proj, geom, D, pw, out_dtype=torch.float32, weights=None):
-    """Cosine pre-weighting and ramp filtering of a projection stack on the device.
-
-    The ramp is the reference's (cycles per pixel) divided by pw, so that FDK returns the
-    attenuation per voxel (the reference's fbp() returns pw times it). FP32 FFT via torch.
-    """
-    _need_device(proj, "proj")
-    nt, nh, nw = proj.shape
-    if weights is None:
-        weights = fdk_cosine_weights(geom, D, pw, device=proj.device)
-    H = ramp_response(nw, proj.device) / pw
-    q = proj.float() * weights
-    spec = torch.fft.rfft(q, n=2 * nw, dim=2)
-    spec *= H
-    q = torch.fft.irfft(spec, n=2 * nw, dim=2)[..., :nw]
-    return q.to(out_dtype).contiguous()
-
-
-def fdk(proj, geom, D, pw, out=None, store_dtype=torch.float32, weights=None):
+def fdk(proj, geom, D, pw, out=None, store_dtype=torch.float32):
     """Full-scan FDK: filter then FDK-weighted backprojection with norm = 1 / nviews."""
-    q = fdk_filter(proj, geom, D, pw, out_dtype=store_dtype, weights=weights)
+    q = fdk_filter(proj, geom, D, pw, out_dtype=store_dtype)
     norm = torch.full((geom.nviews,), 1.0 / geom.nviews, dtype=torch.float32, device=proj.device)
     return backproject(q, geom, _native.FC_BP_FDK, out=out, norm=norm)

@@ -83,6 +83,9 @@ enum fc_bp_mode { FC_BP_UNWEIGHTED = 0, FC_BP_KATSEVICH = 1, FC_BP_FDK = 2 };
 
 enum fc_detector { FC_DET_FLAT = 0, FC_DET_CYLINDER = 1 };
 
+/* Row-filter kernels (internal selector of the FFT row filter). */
+enum fc_filter_kind { FC_FILTER_RAMP = 0, FC_FILTER_HILBERT = 1 };
+
 /* Per-view geometry record: FC_VIEW_DOUBLES doubles per view, FP64, voxel coordinates
  * (a voxel with global index g along an axis of N voxels has coordinate g - N/2).
  *   [0..2]   source position
@@ -136,7 +139,9 @@ int fc_project_dev(int device, const void* vol, int vol_dtype, const int vol_sha
                    const double* views, double step, const float* norm, void* stream);
 
 /* Exact transpose of fc_project_dev (matched adjoint): vol += A^T proj. vol float32, proj dtype
- * fc_dtype. Float atomics: summation order is not deterministic. */
+ * fc_dtype. Accumulated in 64-bit fixed point (resolution 2^-28 of max |proj * norm * step|) per
+ * 16^3 brick, then one float atomic per voxel: summation order is not deterministic. A NaN or
+ * infinite datum on a ray that reaches the block makes the whole output block NaN. */
 int fc_adjoint_dev(int device, float* vol, const int vol_shape[3], const double origin[3],
                    const void* proj, int proj_dtype, const int proj_shape[3], const double* views,
                    double step, const float* norm, void* stream);
@@ -148,6 +153,21 @@ int fc_adjoint_dev(int device, float* vol, const int vol_shape[3], const double 
 int fc_backproject_dev(int device, float* vol, const int vol_shape[3], const double origin[3],
                        const void* proj, int proj_dtype, const int proj_shape[3],
                        const double* views, const float* norm, int mode, void* stream);
+
+/* FDK pre-processing on the device (replaces the host-side cosine weight and rampfilter of the
+ * reference's fbp, fastcone.py:260-292): every detector row is multiplied by the cosine pre-weight
+ * and convolved with the reference's ramp, i.e. zero-padded to 2 Nw with the data at [0, Nw), its
+ * DFT multiplied by H[k] = ramp_scale * pi * k / (2 Nw), H[0] = ramp_scale * pi / (16 Nw),
+ * transformed back and cropped to [0, Nw). The weight is the cosine of the angle between the
+ * pixel's ray and the detector normal, from the view records (physical units; flat-detector
+ * records, kind FC_DET_FLAT; other kinds give NaN); views == NULL skips it (pure ramp).
+ * proj: device [nviews][nh][nw] dtype proj_dtype; out: device, same shape, dtype out_dtype; out
+ * may equal proj (in place). proj_shape = [nw, nh, nviews], 1 <= nw <= FC_FILTER_MAX_COLS.
+ * FP32 arithmetic (FFT of length 2^p >= 2 Nw - 1 in LDS). */
+#define FC_FILTER_MAX_COLS 4096
+int fc_fdk_filter_dev(int device, const void* proj, int proj_dtype, void* out, int out_dtype,
+                      const int proj_shape[3], const double* views, double ramp_scale,
+                      void* stream);
 
 /* Katsevich exact helical reconstruction on a cylindrical detector (no reference counterpart: the
  * reference offers only a U-weighted backprojection of pre-filtered data, GPU_CB_K_backproject

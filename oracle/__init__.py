@@ -7,7 +7,8 @@ only as the checker (or the timed CPU baseline), never as the product path.
 Contents
   * ``liboracle.so`` (oracle.cpp, C++/OpenMP FP64): geometry, forward projection at step h,
     FDK/Katsevich/unweighted backprojection and the exact transpose of the projector;
-  * numpy FP64 restatements of ``fastcone.rampfilter`` and of the ``fastcone.fbp`` cosine weight;
+  * numpy FP64 restatements of ``fastcone.rampfilter``, of the ``fastcone.fbp`` cosine weight and
+    of the physical-units FDK pre-processing (cosine weight from the geometry vectors + ramp / pw);
   * a scipy ``map_coordinates`` brute force for tiny problems (a second, independent
     implementation of the same interpolants: order-1 splines with ``mode="grid-constant"`` are the
     zero-extended trilinear / bilinear interpolants);
@@ -235,6 +236,36 @@ def fbp_cosine_weight(shape, D, zorigin):
         hh = np.arange(nh, dtype=np.float64)[:, None] - float(zorigin[t])
         out[t] = D / np.sqrt(D * D + hh * hh + ww[None, :] ** 2)
     return out
+
+
+def fdk_cosine_weight(prob, window=None):
+    """Physical-units FDK pre-weight of every window pixel: the cosine of the angle between the
+    pixel's ray and the detector normal, |(det - src) . n| / |P - src| with n the unit normal of
+    the detector plane spanned by wstep and hstep, P = det + gw wstep + gh hstep and
+    gw = j + woff - Nw//2, gh = i + hoff - Nh//2 (the pixel convention of coneBeamGPU.cu:523-524).
+    For the aligned geometry this is D / sqrt(D^2 + (gw pw)^2 + (gh pw)^2): the weight of
+    fastcone.py:261-266 with the pixel offsets scaled by the pitch. Returns [n, nh, nw] FP64."""
+    nw, nh = (prob.proj_dim if window is None else window)
+    Nw, Nh = int(prob.proj_dim[0]), int(prob.proj_dim[1])
+    out = np.empty((prob.n, int(nh), int(nw)))
+    jj, ii = np.meshgrid(np.arange(int(nw)), np.arange(int(nh)))
+    for t in range(prob.n):
+        s, d, ws, hs = prob.src[t], prob.det[t], prob.wstep[t], prob.hstep[t]
+        nrm = np.cross(ws, hs)
+        nrm = nrm / np.linalg.norm(nrm)
+        gw = jj + prob.woff[t] - Nw // 2
+        gh = ii + prob.hoff[t] - Nh // 2
+        P = d[None, None, :] + gw[..., None] * ws + gh[..., None] * hs
+        out[t] = abs(float(np.dot(d - s, nrm))) / np.linalg.norm(P - s, axis=-1)
+    return out
+
+
+def fdk_filter(prob, proj, window=None, pw=None):
+    """FDK pre-processing in physical units: cosine pre-weight (fdk_cosine_weight), then the
+    reference's ramp (rampfilter, fastcone.py:272-292) divided by the pitch pw (default the
+    problem's), so that FDK returns the attenuation per voxel. FP64."""
+    pw = prob.pw if pw is None else float(pw)
+    return rampfilter(np.asarray(proj, np.float64) * fdk_cosine_weight(prob, window)) / pw
 
 
 # ----------------------------------------------------------------------------------------------

@@ -130,12 +130,37 @@ def test_abi_reports_bad_arguments_without_touching_memory():
                      ip([10, 10]), woff, hoff, norm)
     assert L.fc_last_error() == _native.FC_ERR_ARGUMENT
     assert b"empty volume block" in L.fc_last_error_message()
-    # source inside the volume
-    base[5] = ctypes.c_float(3.0)
+    # non-positive source distance
+    base[5] = ctypes.c_float(0.0)
     L.fc_clear_error()
     L.GPU_CB_K_backproject(*base, ip([8, 8, 8]), ip([0, 0, 0]), ip([8, 8, 8]), ip([10, 10, 3]),
                            ip([10, 10]), woff, hoff, norm, ctypes.c_int(2))
     assert L.fc_last_error() == _native.FC_ERR_ARGUMENT
+    assert b"sourceSampleDistance" in L.fc_last_error_message()
+    assert np.all(v == 0) and np.all(p == 0)
+
+
+@pytest.mark.skipif(gpu_available(), reason="checks the validation order on a host without GPU")
+def test_abi_accepts_blocks_that_reach_the_source():
+    # The reference projects and backprojects blocks whose corners lie beyond the source radius
+    # (its sweep start may go negative, coneBeamGPU.cu:1781): validation must pass them, so on a
+    # host without a GPU the only error left is the missing device.
+    L = _native.lib()
+    v, p, ang, hm = _abi_args()
+    ip = lambda a: (ctypes.c_int * len(a))(*a)
+    woff, hoff = ip([0, 0, 0]), ip([0, 0, 0])
+    norm = (ctypes.c_float * 3)(1, 1, 1)
+    fp = ctypes.POINTER(ctypes.c_float)
+    base = [ctypes.c_int(0), v.ctypes.data_as(fp), p.ctypes.data_as(fp), ang.ctypes.data_as(fp),
+            hm.ctypes.data_as(fp), ctypes.c_float(3.0), ctypes.c_float(1.0)]
+    dims = [ip([8, 8, 8]), ip([0, 0, 0]), ip([8, 8, 8]), ip([10, 10, 3]), ip([10, 10]), woff,
+            hoff, norm]
+    for call in (lambda: L.GPU_CB_project(*base, *dims),
+                 lambda: L.GPU_CB_K_backproject(*base, *dims, ctypes.c_int(2)),
+                 lambda: L.GPU_CB_misalign_project(*base, *dims, (ctypes.c_float * 6)())):
+        L.fc_clear_error()
+        call()
+        assert L.fc_last_error() == _native.FC_ERR_DEVICE, L.fc_last_error_message()
     assert np.all(v == 0) and np.all(p == 0)
 
 
@@ -207,25 +232,3 @@ def test_windowed_ramp_filters():
     assert hi(c) < 0.5 * hi(plain)
     with pytest.raises(ValueError):
         fastcone.ramp_response(8, "hann")
-
-
-def test_fdk_cosine_weights_are_the_ray_normal_cosine():
-    # upright circular geometry: the detector normal is the horizontal direction from the source
-    # to the rotation axis, (cos a, sin a, 0) up to sign, fixed by the view angle alone
-    import torch
-    from fastcone_amd import ops
-    from fastcone_amd.geometry import Geometry
-    ang = np.array([0.0, 0.7, 2.9, 4.4])
-    g = Geometry.circular(ang, 40.0, 0.8, (10, 12, 14), (9, 11), hmid=[5.0, 4.0, 6.5, 5.0],
-                          woff=[0, 1, -2, 3], hoff=[0, 2, 1, -1], det_dim=(12, 14))
-    w = ops.fdk_cosine_weights(g, 40.0, 0.8, chunk=3).numpy()
-    v = g.views
-    nh, nw = g.det_shape
-    jj, ii = np.meshgrid(np.arange(nw), np.arange(nh))
-    for t, a in enumerate(ang):
-        P = v[t, 3:6] + jj[..., None] * v[t, 6:9] + ii[..., None] * v[t, 9:12]
-        r = P - v[t, 0:3]
-        n = np.array([np.cos(a), np.sin(a), 0.0])
-        ref = np.abs(r @ n) / np.linalg.norm(r, axis=-1)
-        assert np.allclose(w[t], ref, rtol=0, atol=2e-7)
-    assert w.max() <= 1.0 and w.min() > 0.5

@@ -56,13 +56,237 @@ def test_backproject_mode0_misaligned_matches_bruteforce():
 
 
 def test_mode0_equals_unweighted_fdk_geometry_when_aligned():
-    # with no misalignment the general intersection reduces to the FDK coordinates
+    # With no misalignment the general source-voxel-ray / detector-plane intersection of mode 0
+    # (coneBeamGPU.cu:235-243) reduces to the FDK coordinates of coneBeamGPU.cu:398-399,
+    # w = (y cos t - x sin t) U / pw, h = (z - zc) U / pw, U = D / (D + x cos t + y sin t), sampled
+    # with no weight. Written here from the angles alone (no geometry vectors, no oracle code).
+    from scipy.ndimage import map_coordinates
     rng = np.random.default_rng(4)
     prob = _prob(rng)
     p = rng.standard_normal((7, 17, 19))
-    a0 = oracle.backproject(prob, p, (9, 10, 12), mode=0)
-    b0 = oracle.backproject_bruteforce(prob, p, (9, 10, 12), mode=0)
-    assert np.allclose(a0, b0, rtol=0, atol=1e-12)
+    shape = (9, 10, 12)
+    a0 = oracle.backproject(prob, p, shape, mode=0)
+    Nw, Nh = prob.proj_dim
+    o = prob.origin
+    zz, yy, xx = np.meshgrid(*(np.arange(n, dtype=np.float64) for n in shape), indexing="ij")
+    x, y, z = xx + o[0], yy + o[1], zz + o[2]
+    ref = np.zeros(shape)
+    for t, th in enumerate(prob.angles):
+        U = prob.D / (prob.D + x * np.cos(th) + y * np.sin(th))
+        w = (y * np.cos(th) - x * np.sin(th)) * U / prob.pw + Nw // 2 - prob.woff[t]
+        h = (z - prob.zc[t]) * U / prob.pw + Nh // 2 - prob.hoff[t]
+        ref += (1.0 / prob.n) * map_coordinates(p[t], np.stack([h.ravel(), w.ravel()]), order=1,
+                                                mode="grid-constant", cval=0.0).reshape(shape)
+    assert np.abs(a0 - ref).max() <= 1e-12 * np.abs(ref).max()
+    # and the FDK mode is the same sampling weighted by U^2 (mode 1: U)
+    a2 = oracle.backproject(prob, p, shape, mode=2)
+    a1 = oracle.backproject(prob, p, shape, mode=1)
+    ref2 = np.zeros(shape)
+    ref1 = np.zeros(shape)
+    for t, th in enumerate(prob.angles):
+        U = prob.D / (prob.D + x * np.cos(th) + y * np.sin(th))
+        w = (y * np.cos(th) - x * np.sin(th)) * U / prob.pw + Nw // 2 - prob.woff[t]
+        h = (z - prob.zc[t]) * U / prob.pw + Nh // 2 - prob.hoff[t]
+        b = map_coordinates(p[t], np.stack([h.ravel(), w.ravel()]), order=1,
+                            mode="grid-constant", cval=0.0).reshape(shape)
+        ref2 += (1.0 / prob.n) * U * U * b
+        ref1 += (1.0 / prob.n) * U * b
+    assert np.abs(a2 - ref2).max() <= 1e-12 * np.abs(ref2).max()
+    assert np.abs(a1 - ref1).max() <= 1e-12 * np.abs(ref1).max()
+
+
+# ---- the six-parameter misalignment chain (coneBeamGPU.cu:1701-1718) ------------------------------
+# Each parameter alone is checked against what the mathematics fixes, with no oracle code: an
+# integer detector shift is a window offset, Dl is a magnification, Dphi / Dtheta are rotations
+# about the named axis (scipy's Rotation, in the sense the reference's cross products give), and
+# Dpsi is the reference's in-plane "rotation", which the same identities show to be a skew.
+
+def _geom(angles, D=40.0, pw=0.8, mis=None, hmid=6.0, nz=12):
+    angles = np.asarray(angles, np.float64)
+    return oracle.geometry(angles, np.full(angles.size, hmid), D, pw, nz, mis)
+
+
+def _unit(v):
+    return v / np.linalg.norm(v, axis=-1, keepdims=True)
+
+
+def test_misalignment_aligned_vectors_closed_form():
+    th = np.array([0.0, 0.4, 2.2, 5.0])
+    src, det, ws, hs = _geom(th, D=40.0, pw=0.8, hmid=6.0, nz=12)
+    zc = 6.0 - 12 // 2
+    assert np.allclose(src, np.stack([-40 * np.cos(th), -40 * np.sin(th), np.full(4, zc)], 1))
+    assert np.allclose(det, np.stack([0 * th, 0 * th, np.full(4, zc)], 1))
+    assert np.allclose(ws, 0.8 * np.stack([-np.sin(th), np.cos(th), 0 * th], 1))
+    assert np.allclose(hs, 0.8 * np.stack([0 * th, 0 * th, 1 + 0 * th], 1))
+
+
+@pytest.mark.parametrize("axis,k", [(0, 3), (0, -2), (1, 2), (1, -4)])
+def test_misalignment_integer_shift_equals_window_offset(axis, k):
+    # Dw (Dh) of k pixels moves the detector centre by k w-steps (h-steps): window pixel j of the
+    # shifted detector is pixel j + k of the unshifted one
+    rng = np.random.default_rng(20 + axis)
+    vol = rng.standard_normal((9, 10, 12))
+    mis = np.zeros(6)
+    mis[axis] = k
+    a = _prob(rng, mis=mis)
+    woff, hoff = a.woff.copy(), a.hoff.copy()
+    if axis == 0:
+        woff = woff + k
+    else:
+        hoff = hoff + k
+    b = oracle.Problem(a.angles, a.hmid, a.D, a.pw, vol_dim=a.vol_dim, vol_start=a.vol_start,
+                       proj_dim=a.proj_dim, woff=woff, hoff=hoff)
+    pa = oracle.project(a, vol, window=(19, 17), h=0.5)
+    pb = oracle.project(b, vol, window=(19, 17), h=0.5)
+    assert np.abs(pa - pb).max() <= 1e-12 * np.abs(pb).max()
+
+
+def test_misalignment_Dl_is_a_magnification():
+    # the detector moved Dl further from the source along the central ray: every pixel ray equals
+    # that of the aligned detector with pitch pw * D / (D + Dl)
+    rng = np.random.default_rng(22)
+    vol = rng.standard_normal((9, 10, 12))
+    D, pw, Dl = 30.0, 0.8, 7.5
+    a = _prob(rng, mis=[0, 0, Dl, 0, 0, 0], D=D, pw=pw)
+    b = oracle.Problem(a.angles, a.hmid, D, pw * D / (D + Dl), vol_dim=a.vol_dim,
+                       vol_start=a.vol_start, proj_dim=a.proj_dim, woff=a.woff, hoff=a.hoff)
+    pa = oracle.project(a, vol, window=(19, 17), h=0.5)
+    pb = oracle.project(b, vol, window=(19, 17), h=0.5)
+    assert np.abs(pa - pb).max() <= 1e-11 * np.abs(pb).max()
+    src, det, ws, hs = _geom(a.angles, D=D, pw=pw, mis=[0, 0, Dl, 0, 0, 0])
+    assert np.allclose(np.linalg.norm(det - src, axis=1), D + Dl)
+
+
+def test_misalignment_Dl_follows_the_shifted_centre():
+    # Dl moves the (already Dw/Dh-shifted) centre along the source -> centre line
+    th = np.array([0.3, 1.9])
+    mis = [2.0, -1.0, 5.0, 0, 0, 0]
+    src, det, ws, hs = _geom(th, mis=mis)
+    src0, det0, ws0, hs0 = _geom(th)
+    shifted = det0 + 2.0 * ws0 - 1.0 * hs0
+    assert np.allclose(np.linalg.norm(det - src, axis=1), np.linalg.norm(shifted - src, axis=1) + 5.0)
+    assert np.allclose(_unit(det - src), _unit(shifted - src))
+
+
+def test_misalignment_Dphi_rotates_h_about_w():
+    from scipy.spatial.transform import Rotation
+    th = np.array([0.0, 0.7, 3.5])
+    phi = 0.21
+    src0, det0, ws0, hs0 = _geom(th)
+    src, det, ws, hs = _geom(th, mis=[0, 0, 0, phi, 0, 0])
+    for t in range(th.size):
+        R = Rotation.from_rotvec(phi * _unit(ws0[t]))
+        assert np.allclose(hs[t], R.apply(hs0[t]), atol=1e-12)
+        assert np.allclose(ws[t], ws0[t], atol=1e-12)
+    assert np.allclose(src, src0) and np.allclose(det, det0)
+    # positive Dphi tilts h away from the source
+    assert np.all(np.einsum("ij,ij->i", hs, det0 - src0) > 0)
+
+
+def test_misalignment_Dtheta_rotates_w_about_h():
+    from scipy.spatial.transform import Rotation
+    th = np.array([0.0, 0.7, 3.5])
+    tt = -0.17
+    src0, det0, ws0, hs0 = _geom(th)
+    src, det, ws, hs = _geom(th, mis=[0, 0, 0, 0, tt, 0])
+    for t in range(th.size):
+        # w <- cos(tt) w + sin(tt) (w x h)^: a rotation about h by -tt
+        R = Rotation.from_rotvec(-tt * _unit(hs0[t]))
+        assert np.allclose(ws[t], R.apply(ws0[t]), atol=1e-12)
+        assert np.allclose(hs[t], hs0[t], atol=1e-12)
+
+
+def test_misalignment_Dpsi_is_a_skew_of_w_and_h():
+    # w <- cos(psi) w - sin(psi) (w x n)^ and h <- cos(psi) h + sin(psi) (h x n)^, n = w x h:
+    # for orthonormal w, h the identities a x (b x c) = b (a.c) - c (a.b) give (w x n)^ = -h and
+    # (h x n)^ = w, so both vectors tilt towards each other (not a rotation: w'.h' = sin 2 psi)
+    th = np.array([0.2, 2.6])
+    psi = 0.13
+    src0, det0, ws0, hs0 = _geom(th, pw=0.8)
+    src, det, ws, hs = _geom(th, pw=0.8, mis=[0, 0, 0, 0, 0, psi])
+    w, h = _unit(ws0), _unit(hs0)
+    assert np.allclose(ws, 0.8 * (np.cos(psi) * w + np.sin(psi) * h), atol=1e-12)
+    assert np.allclose(hs, 0.8 * (np.sin(psi) * w + np.cos(psi) * h), atol=1e-12)
+    assert np.allclose(np.einsum("ij,ij->i", _unit(ws), _unit(hs)), np.sin(2 * psi))
+
+
+def test_misalignment_full_chain_composes_in_order():
+    # Dw, Dh, Dl on the centre; then Dphi (h about w), Dtheta (w about the new h), Dpsi (skew of
+    # the resulting pair): composed with scipy rotations, against the oracle's chain
+    from scipy.spatial.transform import Rotation
+    th = np.array([0.5, 4.1])
+    mis = np.array([0.6, -0.8, 1.5, 0.09, -0.12, 0.07])
+    src0, det0, ws0, hs0 = _geom(th, D=50.0, pw=0.9)
+    src, det, ws, hs = _geom(th, D=50.0, pw=0.9, mis=mis)
+    for t in range(th.size):
+        d = det0[t] + mis[0] * ws0[t] + mis[1] * hs0[t]
+        d = d + mis[2] * _unit(d - src0[t])
+        h1 = Rotation.from_rotvec(mis[3] * _unit(ws0[t])).apply(hs0[t])
+        w1 = Rotation.from_rotvec(-mis[4] * _unit(h1)).apply(ws0[t])
+        w2 = 0.9 * (np.cos(mis[5]) * _unit(w1) + np.sin(mis[5]) * _unit(h1))
+        h2 = 0.9 * (np.sin(mis[5]) * _unit(w1) + np.cos(mis[5]) * _unit(h1))
+        assert np.allclose(src[t], src0[t], atol=1e-12)
+        assert np.allclose(det[t], d, atol=1e-12)
+        assert np.allclose(ws[t], w2, atol=1e-12)
+        assert np.allclose(hs[t], h2, atol=1e-12)
+    # pitch lengths are preserved by every step
+    assert np.allclose(np.linalg.norm(ws, axis=1), 0.9)
+    assert np.allclose(np.linalg.norm(hs, axis=1), 0.9)
+
+
+# ---- physical-units FDK pre-processing ------------------------------------------------------------
+
+def test_fdk_cosine_weight_closed_form_when_aligned():
+    # aligned detector through the axis: the weight is D / sqrt(D^2 + (gw pw)^2 + (gh pw)^2) with
+    # the global pixel offsets gw = j + woff - Nw//2, gh = i + hoff - Nh//2
+    rng = np.random.default_rng(30)
+    prob = _prob(rng, D=30.0, pw=0.8)
+    w = oracle.fdk_cosine_weight(prob, window=(19, 17))
+    Nw, Nh = prob.proj_dim
+    for t in range(prob.n):
+        gw = np.arange(19) + prob.woff[t] - Nw // 2
+        gh = np.arange(17) + prob.hoff[t] - Nh // 2
+        ref = 30.0 / np.sqrt(30.0 ** 2 + (0.8 * gw[None, :]) ** 2 + (0.8 * gh[:, None]) ** 2)
+        assert np.allclose(w[t], ref, rtol=0, atol=1e-14)
+
+
+def test_fdk_cosine_weight_tilted_detector_is_the_ray_normal_cosine():
+    # tilted detector: w |P - src| is the same for every pixel (the source's distance from the
+    # detector plane, here from a least-squares fit of the plane through the pixel centres)
+    rng = np.random.default_rng(31)
+    prob = _prob(rng, mis=[0.7, -1.2, 2.0, 0.2, -0.25, 0.0])
+    w = oracle.fdk_cosine_weight(prob, window=(19, 17))
+    Nw, Nh = prob.proj_dim
+    for t in range(prob.n):
+        jj, ii = np.meshgrid(np.arange(19) + prob.woff[t] - Nw // 2,
+                             np.arange(17) + prob.hoff[t] - Nh // 2)
+        P = prob.det[t] + jj[..., None] * prob.wstep[t] + ii[..., None] * prob.hstep[t]
+        Q = P.reshape(-1, 3)
+        c = Q.mean(0)
+        nrm = np.linalg.svd(Q - c)[2][-1]
+        dist = abs(np.dot(c - prob.src[t], nrm))
+        assert np.allclose(w[t] * np.linalg.norm(P - prob.src[t], axis=-1), dist, rtol=1e-12)
+        assert w[t].max() <= 1.0
+
+
+def test_fdk_filter_is_weight_then_ramp_over_pw():
+    # linear in the data; an impulse in row i, column m gives weight(i, m) / pw times the ramp's
+    # impulse response shifted to m (the explicit inverse DFT of the 2 Nw ramp)
+    rng = np.random.default_rng(32)
+    prob = _prob(rng, n=2, D=30.0, pw=0.8)
+    nw = 19
+    p = np.zeros((2, 17, nw))
+    p[1, 5, 7] = 1.0
+    q = oracle.fdk_filter(prob, p, window=(nw, 17))
+    w = oracle.fdk_cosine_weight(prob, window=(nw, 17))
+    n = 2 * nw
+    H = oracle.ramp_filter_response(nw)
+    Hf = np.concatenate([H, H[1:-1][::-1]])
+    k = np.arange(n)
+    h = np.array([np.sum(Hf * np.cos(2 * np.pi * k * d / n)) / n for d in range(n)])
+    ref = w[1, 5, 7] / 0.8 * h[(np.arange(nw) - 7) % n]
+    assert np.allclose(q[1, 5], ref, atol=1e-14)
+    assert np.abs(q[0]).max() == 0.0 and np.abs(np.delete(q[1], 5, axis=0)).max() == 0.0
 
 
 @pytest.mark.parametrize("h", [1.0, 0.5])
