@@ -51,6 +51,7 @@ def _declare(lib):
         "fc_sample_count_dev": (i, [i, _c_int_p, _c_dbl_p, _c_int_p, _vp, d, _vp, _vp]),
         "fc_convert_dev": (i, [i, _vp, i, _vp, i, ll, _vp]),
         "fc_fdk_filter_dev": (i, [i, _vp, i, _vp, i, _c_int_p, _vp, d, _vp]),
+        "fc_fdk_filter_noalloc_dev": (i, [i, _vp, i, _vp, i, _c_int_p, _vp, d, _vp]),
         "fc_conv3d_fwd_dev": (i, [i, _vp, _vp, _vp, _vp, i, i, i, i, i, i, i, _vp]),
         "fc_conv3d_dgrad_dev": (i, [i, _vp, _vp, _vp, _vp, i, i, i, i, i, i, _vp]),
         "fc_conv3d_wgrad_dev": (i, [i, _vp, _vp, _vp, i, i, i, i, i, i, _vp]),

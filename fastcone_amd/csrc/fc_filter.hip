@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 
 #include "fc_device.h"
 #include "fc_internal.h"
@@ -307,11 +308,37 @@ int filter_log2(int nw) {
 
 inline constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+// Tables of the widest supported row (FC_FILTER_MAX_COLS columns, FFT length 8192): twiddles and
+// multipliers (8192 float2 each) and the 8191 FP64 taps. Every row filter needs at most this.
+constexpr size_t kMaxTableBytes = 2 * align256((size_t)8192 * sizeof(float2)) +
+                                  align256((2 * (size_t)FC_FILTER_MAX_COLS - 1) * sizeof(double));
+
+// Device-resident tables for fc_fdk_filter_noalloc_dev (one copy per GPU, held until the process
+// ends). In host code the name is only a stand-in: the device address comes from
+// hipGetSymbolAddress, once per device.
+static __device__ __align__(256) unsigned char scratch[kMaxTableBytes];
+
 template <int LOG2L>
 void launch_rows(const FiltArgs& a, hipStream_t s) {
   const long long npairs = (a.nrows + 1) / 2;
   const int blocks = (int)(npairs < 2048 ? npairs : 2048);
   row_filter_kernel<LOG2L><<<blocks, kFT, 0, s>>>(a);
+}
+
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[64] = {nullptr};
+
+// The calling thread's current device's address of `scratch`, looked up on first use.
+int scratch_address(int device, void** out) {
+  FC_CHECK_ARG(device >= 0 && device < 64, "device id %d too large", device);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    FC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return FC_OK;
 }
 
 }  // namespace
@@ -392,6 +419,33 @@ int fc_fdk_filter_dev(int device, const void* proj, int proj_dtype, void* out, i
   prof_end(tok, s);
   if (rc) return rc;
   return release_ws(device, SLOT_FILTER, s);
+}
+
+// The same filter with no device allocation: its tables live in the file-scope `scratch` array
+// (kMaxTableBytes, enough for FC_FILTER_MAX_COLS columns) instead of a library workspace. No
+// counterpart in the reference.
+int fc_fdk_filter_noalloc_dev(int device, const void* proj, int proj_dtype, void* out,
+                              int out_dtype, const int proj_shape[3], const double* views,
+                              double ramp_scale, void* stream) {
+  clear_error();
+  FC_CHECK_ARG(proj_shape, "null shape");
+  const int nw = proj_shape[0], nh = proj_shape[1], nv = proj_shape[2];
+  FC_CHECK_ARG(nw >= 1 && nw <= FC_FILTER_MAX_COLS,
+               "detector width %d out of [1, %d] (the no-allocation tables hold %zu bytes)", nw,
+               FC_FILTER_MAX_COLS, kMaxTableBytes);
+  FC_CHECK_ARG(filter_table_bytes(nw) <= kMaxTableBytes, "filter tables exceed the scratch array");
+  FC_CHECK_ARG(nh >= 1 && nv >= 0, "projection shape invalid (%d,%d,%d)", nw, nh, nv);
+  FC_CHECK_ARG(dtype_ok(proj_dtype) && dtype_ok(out_dtype), "unknown dtype");
+  FC_CHECK_ARG(std::isfinite(ramp_scale), "ramp scale not finite");
+  FC_CHECK_ARG(nv == 0 || (proj && out), "null pointer argument");
+  int rc = select_device(device);
+  if (rc) return rc;
+  void* tables = nullptr;
+  if ((rc = scratch_address(device, &tables))) return rc;
+  if (nv == 0) return FC_OK;
+  // every table entry the filter reads is rewritten by this call's table kernels (same stream)
+  return launch_row_filter(proj, proj_dtype, out, out_dtype, nw, nh, (long long)nh * nv, views,
+                           FC_FILTER_RAMP, ramp_scale, tables, static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

@@ -169,6 +169,17 @@ int fc_fdk_filter_dev(int device, const void* proj, int proj_dtype, void* out, i
                       const int proj_shape[3], const double* views, double ramp_scale,
                       void* stream);
 
+/* fc_fdk_filter_dev without any device allocation, for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops). No counterpart in the reference. The filter's tables
+ * (twiddles, multipliers, taps: at most 192 KiB, enough for FC_FILTER_MAX_COLS columns) live in a
+ * static device array, one per GPU, held until the process ends; widths beyond FC_FILTER_MAX_COLS
+ * are rejected (FC_ERR_ARGUMENT). Every call rewrites the tables it reads, on its stream; calls
+ * that overlap on one GPU (different streams or threads) share the array and race. Same results
+ * as fc_fdk_filter_dev. */
+int fc_fdk_filter_noalloc_dev(int device, const void* proj, int proj_dtype, void* out,
+                              int out_dtype, const int proj_shape[3], const double* views,
+                              double ramp_scale, void* stream);
+
 /* Katsevich exact helical reconstruction on a cylindrical detector (no reference counterpart: the
  * reference offers only a U-weighted backprojection of pre-filtered data, GPU_CB_K_backproject
  * mode 1, coneBeamGPU.cu:258-338). proj: device [nviews][nh][nw] line integrals (dtype fc_dtype)
