@@ -1,7 +1,12 @@
 """Hand-written conv3d kernels (MFMA) against torch.nn.functional.conv3d in FP32 on the CPU, fed
 the same BF16-rounded inputs and weights. Tolerance: FP32 accumulation in a different order, and
-BF16 rounding of the outputs (relative 2^-8), so rel. L2 < 1e-2 (outputs) and < 2e-3 (FP32
-weight gradients)."""
+BF16 rounding of the outputs (relative 2^-8), so rel. L2 < 1e-2 (outputs and input gradients) and
+< 2e-3 (FP32 weight and bias gradients).
+
+The whole network: its forward and backward passes against torch autograd in FP32 on the CPU with
+the network's BF16 rounding points reproduced (inputs, weights, every activation and the
+gradient at every convolution output), and a short training run that fits a target the network
+can represent."""
 import numpy as np
 import pytest
 
@@ -70,11 +75,15 @@ def test_conv_forward_backward(gpu, cin, cout):
     gy_k = gy.to(torch.bfloat16).float() if cout != 1 else gy
     (y.float() * gy_k.to(gpu)).sum().backward()
     (yr_c * gy_k).sum().backward()
-    # weight gradient from BF16 inputs: compare against the reference with the same rounding
-    assert rel(wg.grad.cpu(), wr.grad.permute(0, 2, 3, 4, 1).reshape(cout, 27, cin)) < 2e-2
-    assert rel(bg.grad.cpu(), br.grad) < 2e-2
+    # weight and bias gradients from BF16 inputs and output gradients (products exact in FP32,
+    # FP32 sums): only the summation order differs from the reference with the same rounding
+    e_w = rel(wg.grad.cpu(), wr.grad.permute(0, 2, 3, 4, 1).reshape(cout, 27, cin))
+    e_b = rel(bg.grad.cpu(), br.grad)
+    print(f"conv {cin}->{cout}: weight gradient {e_w:.1e}, bias gradient {e_b:.1e}")
+    assert e_w < 2e-3 and e_b < 2e-3
     if cin != 1:
-        assert rel(xg.grad.float().cpu(), xr.grad.permute(0, 2, 3, 4, 1)) < 2e-2
+        # the input gradient is stored in BF16 (relative rounding 2^-9)
+        assert rel(xg.grad.float().cpu(), xr.grad.permute(0, 2, 3, 4, 1)) < 1e-2
 
 
 def test_unet_matches_fp32_reference(gpu):
@@ -127,3 +136,95 @@ def test_weight_gradient_long_rows(gpu, cin, cout):
     (y * gy.float().permute(0, 4, 1, 2, 3)).sum().backward()
     ref = w0.grad.permute(0, 2, 3, 4, 1).reshape(cout, 27, cin)
     assert rel(dw.cpu(), ref) < 2e-3
+
+
+class _RoundBF16(torch.autograd.Function):
+    """Forward: round to BF16; backward: round the gradient to BF16 (the U-Net's activations and
+    the output gradients its convolution kernels read are BF16)."""
+
+    @staticmethod
+    def forward(ctx, x):
+        return x.to(torch.bfloat16).float()
+
+    @staticmethod
+    def backward(ctx, g):
+        return g.to(torch.bfloat16).float()
+
+
+def _reference_unet(net, vol):
+    """FP32 CPU autograd restatement of UNet3D.forward with the same BF16 rounding points;
+    returns (output, {layer: weight tensor in torch layout}, {layer: bias})."""
+    from fastcone_amd.unet import UNet3D, LAYERS
+    F = torch.nn.functional
+    W = {k: _torch_w(v.detach().to(torch.bfloat16).float().cpu()).requires_grad_(True)
+         for k, v in net.w.items()}
+    B = {k: v.detach().float().cpu().requires_grad_(True) for k, v in net.b.items()}
+    R = _RoundBF16.apply
+
+    def c(x, n):
+        relu = next(L[3] for L in LAYERS if L[0] == n)
+        y = F.conv3d(x, W[n], B[n], padding=1)
+        return R(torch.relu(y)) if relu else y
+
+    x = R(vol.cpu() * UNet3D.SCALE)[:, None]
+    e1 = c(c(x, "e1a"), "e1b")
+    e2 = c(c(R(F.avg_pool3d(e1, 2)), "e2a"), "e2b")
+    bt = c(c(R(F.avg_pool3d(e2, 2)), "b_a"), "b_b")
+    up = lambda t: t.repeat_interleave(2, 2).repeat_interleave(2, 3).repeat_interleave(2, 4)
+    d2 = c(c(torch.cat([up(bt), e2], 1), "d2a"), "d2b")
+    d1 = c(c(torch.cat([up(d2), e1], 1), "d1a"), "d1b")
+    out = vol.cpu() + c(d1, "out")[:, 0] / UNet3D.SCALE
+    return out, W, B
+
+
+def test_unet_forward_and_backward_match_fp32_autograd(gpu):
+    # whole-network gradients (pooling, upsampling, skip concatenations, the 1/SCALE residual
+    # output) for the loss sum(out * G) with a fixed random G, so both sides receive the same
+    # output gradient
+    from fastcone_amd.unet import UNet3D
+    net = UNet3D(seed=3).to(gpu)
+    g = torch.Generator().manual_seed(4)
+    vol = (torch.rand((2, 16, 16, 16), generator=g) * 0.02).to(gpu)
+    G = torch.randn((2, 16, 16, 16), generator=g)
+    out = net(vol)
+    (out * G.to(gpu)).sum().backward()
+    ref, W, B = _reference_unet(net, vol)
+    (ref * G).sum().backward()
+    e_fwd = rel((out.detach().cpu() - vol.cpu()).double(), (ref.detach() - vol.cpu()).double())
+    print(f"U-Net correction vs FP32 autograd: {e_fwd:.1e}")
+    assert e_fwd < 1e-2
+    for name in net.w:
+        gw = net.w[name].grad.cpu()
+        gr = W[name].grad.permute(0, 2, 3, 4, 1).reshape(gw.shape)
+        e_w = rel(gw, gr)
+        e_b = rel(net.b[name].grad.cpu(), B[name].grad)
+        print(f"  {name}: weight gradient {e_w:.1e}, bias gradient {e_b:.1e}")
+        assert e_w < 3e-2 and e_b < 3e-2, name
+
+
+def test_unet_training_fits_a_representable_target(gpu):
+    # 8 pairs whose target differs from the input by a fixed local linear map (a 3x3x3 blur
+    # minus identity): a function the network can represent. Adam at the training settings
+    # (lr 1e-3) must bring the training MSE far below the identity start's, which shows that the
+    # kernels' gradients drive training; a target that is not a local function of the input
+    # (amplified noise) is not fitted by any such network
+    from fastcone_amd.unet import UNet3D
+    net = UNet3D(seed=5).to(gpu)
+    g = torch.Generator(device=gpu).manual_seed(6)
+    x = torch.rand((8, 32, 32, 32), device=gpu, generator=g) * 0.02
+    k = torch.tensor([0.25, 0.5, 0.25], device=gpu)
+    blur = (k[:, None, None] * k[None, :, None] * k[None, None, :])[None, None]
+    y = torch.nn.functional.conv3d(x[:, None], blur, padding=1)[:, 0]
+    y = x + 0.5 * (y - x)
+    opt = torch.optim.Adam(net.parameters(), lr=1e-3)
+    mse0 = float(((x - y) ** 2).mean())
+    for it in range(300):
+        idx = torch.arange(4, device=gpu) + 4 * (it % 2)
+        loss = ((net(x[idx]) - y[idx]) ** 2).mean() * UNet3D.SCALE ** 2
+        opt.zero_grad(set_to_none=True)
+        loss.backward()
+        opt.step()
+    with torch.no_grad():
+        mse = float(((net(x) - y) ** 2).mean())
+    print(f"representable target: training MSE {mse / mse0:.3f} of the identity start's")
+    assert mse < 0.2 * mse0

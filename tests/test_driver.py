@@ -80,3 +80,68 @@ def test_chunked_equals_whole(gpu, mode):
     cone.backproject(v_ref, p_ref, ang, D, zs, kernelMode=mode, pixelWidth=PW)
     for t in range(DIMS[0]):
         assert np.linalg.norm(v_drv[t] - v_ref) / np.linalg.norm(v_ref) < 1e-5
+
+
+# ---- the driver against the FP64 oracle directly ---------------------------------------------------
+
+TOTAL = np.array([2, 30, 26, 28])       # t, z, y, x of the whole volume
+LOCAL = np.array([2, 24, 20, 22])       # this GPU's block
+OFFSET = np.array([0, 3, 2, 4])         # its global offset
+
+
+def _oracle_prob(ang, zs, views, mis=None):
+    return oracle.Problem(ang[views], zs[views], D, PW, vol_dim=TOTAL[1:][::-1],
+                          vol_start=OFFSET[1:][::-1], proj_dim=(NW, NH), mis=mis)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mis", [None, [0.5, -0.7, 1.2, 0.03, -0.02, 0.04]])
+def test_simple_project_matches_oracle(gpu, mis):
+    # several chunks, two volumes (t), a sub-block of a larger volume, every second view
+    # (projStride 2, projOffset 1): each selected view receives the sum over t of the oracle's
+    # unit-step projection of the block, except in global column 0 (fastcone.project zeroes it in
+    # every chunk window that starts there, so the caller's column 0 receives nothing); other
+    # views are untouched
+    ang, zs = _setup()
+    rng = np.random.default_rng(2)
+    vol = rng.standard_normal(LOCAL).astype(np.float32)
+    base = rng.standard_normal((NV, NH, NW)).astype(np.float32)
+    p = base.copy()
+    mis32 = np.zeros(6, np.float32) if mis is None else np.asarray(mis, np.float32)
+    driver.simpleProject(None, p, vol, ang, OFFSET, TOTAL, zs, D, chunk=(1, 10, 8, 11),
+                         misAlignments=mis32, pixelWidth=PW, projStride=2, projOffset=1)
+    sel = np.array([v for v in range(NV) if (v + 1) % 2 == 0])
+    prob = _oracle_prob(ang.astype(np.float32).astype(np.float64),
+                        zs.astype(np.float32).astype(np.float64), sel,
+                        None if mis is None else mis32.astype(np.float64))
+    ref = base.astype(np.float64).copy()
+    add = sum(oracle.project(prob, vol[t].astype(np.float64), h=1.0) for t in range(LOCAL[0]))
+    add[:, :, 0] = 0.0
+    ref[sel] += add
+    rest = np.setdiff1d(np.arange(NV), sel)
+    assert np.array_equal(p[rest], base[rest])
+    e = float(np.linalg.norm(p[sel] - ref[sel]) / np.linalg.norm(ref[sel]))
+    assert e < 5e-5, e
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [2, 1, 0])
+def test_simple_backproject_matches_oracle(gpu, mode):
+    # each t volume receives the oracle backprojection of the selected views with norm
+    # 1 / (all views), the reference's backWeightings (mpi_fastcone.py:285)
+    ang, zs = _setup()
+    rng = np.random.default_rng(3)
+    proj = rng.standard_normal((NV, NH, NW)).astype(np.float32)
+    base = rng.standard_normal(LOCAL).astype(np.float32)
+    v = base.copy()
+    driver.simpleBackproject(None, proj, v, ang, OFFSET, TOTAL, zs, D, chunk=(1, 10, 8, 11),
+                             kernelMode=mode, pixelWidth=PW, projStride=3, projOffset=0)
+    sel = np.arange(0, NV, 3)
+    prob = _oracle_prob(ang.astype(np.float32).astype(np.float64),
+                        zs.astype(np.float32).astype(np.float64), sel)
+    ref = oracle.backproject(prob, proj[sel].astype(np.float64), tuple(int(n) for n in LOCAL[1:]),
+                             mode=mode,
+                             norm=np.full(sel.size, 1.0 / NV))
+    for t in range(LOCAL[0]):
+        e = float(np.linalg.norm(v[t] - base[t] - ref) / np.linalg.norm(ref))
+        assert e < 5e-5, (t, e)

@@ -507,3 +507,25 @@ def test_project_random_dense_geometries(gpu, quad, monkeypatch):
         ref = oracle.project(prob, x, window=det[::-1], h=h)
         p = ops.project(torch.from_numpy(x).to(gpu), g, step=h).cpu()
         assert rel(p, ref) < 5e-5, (case, nv, D, pw, vol, det, mis, h)
+
+
+def test_adjoint_of_non_finite_data_is_nan_and_cgls_raises(gpu):
+    # a NaN or infinite datum on a ray that reaches the volume cannot be held by the adjoint's
+    # fixed-point accumulator: the output becomes NaN (not silently zero), and CGLS stops with an
+    # error instead of reporting convergence
+    from fastcone_amd import cgls, ops
+    rng = np.random.default_rng(49)
+    g, prob = _setup(rng)
+    base = torch.from_numpy(rng.standard_normal(g.proj_shape).astype(np.float32)).to(gpu)
+    for bad in (float("nan"), float("inf")):
+        y = base.clone()
+        y[2, 13, 15] = bad   # a central pixel: its ray crosses the block
+        out = ops.adjoint(y, g, step=0.5)
+        torch.cuda.synchronize()
+        assert bool(torch.isnan(out).all()), bad
+    ok = ops.adjoint(base, g, step=0.5)
+    assert bool(torch.isfinite(ok).all())
+    y = base.clone()
+    y[2, 13, 15] = float("nan")
+    with pytest.raises(FloatingPointError):
+        cgls.cgls(lambda x: ops.project(x, g, 0.5), lambda r: ops.adjoint(r, g, 0.5), y, iters=3)
