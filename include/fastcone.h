@@ -234,13 +234,13 @@ int fc_set_tile_limits(long long max_voxels, int max_views);
 /* Gather-roofline microbenchmark: the sustained rate (samples/s) of the kernels' own fetch +
  * interpolation code inside small cache- or LDS-resident bricks.
  * which = 0: projector trilinear sample, octet layout of `dtype` (8 taps per sample), random
- *            positions; 1: the same, plain FP32 layout; 2: backprojector bilinear quad gather of
- *            `dtype` (4 taps per sample), random positions; 3: projector trilinear sample from
- *            LDS bricks at random positions (FP32); 4: the LDS projector's inner loop along
- *            ray-coherent paths through one resident brick (FP32; no refill, no divergence: the
- *            ceiling of the default projector); 5: the backprojector's inner loop (per column one
- *            u per view, 16 z rows) over one cache-resident patch of `dtype` quads (4 taps per
- *            sample). Best over several runs. Synchronous. */
+ *            positions; 1: the same, plain FP32 layout; 2: the backprojector's bilinear sample
+ *            from a resident LDS window at random positions (FP32, 4 taps per sample); 3:
+ *            projector trilinear sample from LDS bricks at random positions (FP32); 4: the LDS
+ *            projector's inner loop along ray-coherent paths through one resident brick (FP32; no
+ *            refill, no divergence: the ceiling of the default projector); 5: the backprojector's
+ *            inner loop (per column one u per view, 16 z rows) over one resident LDS window
+ *            (FP32, 4 taps per sample). Best over several runs. Synchronous. */
 int fc_gather_bench(int device, int which, int dtype, double* samples_per_s);
 
 /* Kernel timing. When enabled, the main kernel of every fc_* / GPU_CB_* call ("project",
