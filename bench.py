@@ -9,9 +9,11 @@ One step = forward projection (stage + ray-march kernel) -> FDK filtering (cosin
 ramp, one FFT kernel) -> FDK backprojection (stage + voxel-driven kernel), all device-resident.
 value = W / step time, W = 512^3 * 720 voxel-view updates (Gvoxel-view/s).
 
-Extra fields: per-kernel rates, the gather roofline of the dominant kernel (the projector), the
-BF16 / FP16 storage variants, and a CPU baseline (the FP64 oracle on the host cores, a bounded
-sample of the same workload, extrapolated) with parity of the GPU result on that sample.
+Extra fields: per-kernel rates, the roofline of the dominant kernel (the projector: its LDS gather
+bytes against the LDS bandwidth, with its inner loop's own ceilings and the HBM view beside), the
+BF16 / FP16 storage variants, a CPU baseline (the FP64 oracle on the host cores, a bounded sample
+of the same workload, extrapolated) with parity of the GPU result on that sample, and the CGLS
+time-to-residual leg of the learned warm start.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu] [--no-variants]
 """
@@ -122,23 +124,24 @@ def main():
         ev_ms = ev0.elapsed_time(ev1) / steps
         fp_n, fp_ms = _native.profile_read("project")
         bp_n, bp_ms = _native.profile_read("backproject")
+        fl_n, fl_ms = _native.profile_read("fdk_filter")
         _native.profile_enable(False)
-        return wall, ev_ms, fp_ms / max(fp_n, 1), bp_ms / max(bp_n, 1)
+        return wall, ev_ms, fp_ms / max(fp_n, 1), bp_ms / max(bp_n, 1), fl_ms / max(fl_n, 1)
 
-    wall, ev_ms, fp_ms, bp_ms = timed(vol, torch.float32, args.steps, args.warmup)
+    wall, ev_ms, fp_ms, bp_ms, flt_ms = timed(vol, torch.float32, args.steps, args.warmup)
     log(f"f32: step {wall * 1e3:.1f} ms (events {ev_ms:.1f}), FP kernel {fp_ms:.1f} ms, "
-        f"BP kernel {bp_ms:.1f} ms")
+        f"filter {flt_ms:.1f} ms, BP kernel {bp_ms:.1f} ms")
 
-    # gather roofline of the dominant kernel (the projector: 8 taps per sample). The peak is the
-    # projector's own LDS inner loop along ray-coherent paths through a resident brick (no
-    # refill, no divergence); random positions in LDS bricks (bank conflicts) and the LDS
-    # bandwidth (128 B/clk/CU x 256 CUs x 2.4 GHz, one 4-byte tap per lane read) are reported
-    # beside it.
+    # Roofline of the dominant kernel (the projector). Its algorithmic traffic is the gather from
+    # the LDS bricks: 8 taps x 4 B = 32 B per ray sample; the ceiling is the LDS bandwidth from the
+    # published figures (128 B per clock per CU, 256 CUs, 2.4 GHz). The kernels' own inner loops
+    # on resident bricks / windows (ray-coherent and random positions) and the HBM view
+    # (compulsory bytes against 8 TB/s, plus the measured FETCH/WRITE bytes) are reported beside.
     peak_fp = _native.gather_bench(4, _native.FC_F32)
     peak_fp_random = _native.gather_bench(3, _native.FC_F32)
     peak_bp = _native.gather_bench(5, _native.FC_F32)         # ray-coherent BP inner loop
     peak_bp_random = _native.gather_bench(2, _native.FC_F32)
-    lds_bw_taps = 128.0 * 256 * 2.4e9 / 4.0
+    lds_bw = 128.0 * 256 * 2.4e9                               # bytes / s
     fp_taps = 8.0 * samples
     bp_taps = 4.0 * W
     fp_rate = fp_taps / (fp_ms * 1e-3)
@@ -147,18 +150,20 @@ def main():
     fp_hbm_bytes = (N + 4) ** 3 * 4 + 2 * 4 * nv * NDET * NDET
     traffic = measured_traffic(nv)
     roofline = {
-        "bound": "gather",
+        "bound": "lds",
         "kernel": "fp_quad_kernel + fp_lds_kernel (ray-march projector, LDS slab bricks shared by "
                   "four-view quads)",
-        "achieved": fp_rate / 1e9, "peak": peak_fp * 8 / 1e9, "unit": "Gtap/s",
-        "frac": fp_rate / (peak_fp * 8),
+        "achieved": fp_rate * 4.0 / 1e9, "peak": lds_bw / 1e9, "unit": "GB/s",
+        "frac": fp_rate * 4.0 / lds_bw,
         "traffic": None if traffic is None else traffic["bytes_per_launch"],
-        "per_unit": "8 taps per ray sample; samples = exact count of the clipped rays",
-        "units_per_launch": samples,
-        "peak_kind": "fc_gather_bench(4): the projector's LDS trilinear inner loop on ray-coherent "
-                     "paths through one resident brick",
-        "ceilings": {"lds_random_positions_Gtap_per_s": peak_fp_random * 8 / 1e9,
-                     "lds_bandwidth_Gtap_per_s": lds_bw_taps / 1e9},
+        "per_unit": "32 B of LDS gather per ray sample (8 trilinear taps x 4 B); samples = exact "
+                    "count of the clipped rays",
+        "units_per_launch": samples, "taps_per_s": fp_rate,
+        "peak_kind": "LDS bandwidth: 128 B/clock/CU x 256 CUs x 2.4 GHz (published figures)",
+        "ceilings": {
+            "inner_loop_ray_coherent_GBps": peak_fp * 32 / 1e9,
+            "inner_loop_ray_coherent_frac": fp_rate / (peak_fp * 8),
+            "inner_loop_random_positions_GBps": peak_fp_random * 32 / 1e9},
         "hbm": {"achieved": fp_hbm_bytes / (fp_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": fp_hbm_bytes / (fp_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                 "bytes_per_launch": fp_hbm_bytes,
@@ -169,9 +174,13 @@ def main():
                "Gtap_per_s": fp_rate / 1e9},
         "bp": {"ms": bp_ms, "Gvoxel_view_per_s": W / (bp_ms * 1e-3) / 1e9,
                "Gtap_per_s": bp_taps / (bp_ms * 1e-3) / 1e9,
+               "lds_GBps": bp_taps * 4.0 / (bp_ms * 1e-3) / 1e9,
+               "lds_frac": bp_taps * 4.0 / (bp_ms * 1e-3) / lds_bw,
                "gather_peak_Gtap_per_s": peak_bp * 4 / 1e9,
                "gather_frac": bp_taps / (bp_ms * 1e-3) / (peak_bp * 4),
                "gather_random_positions_Gtap_per_s": peak_bp_random * 4 / 1e9},
+        "filter": {"ms": flt_ms, "kernel": "row_filter_kernel (cosine weight + ramp, FFT in LDS)",
+                   "hbm_GBps": 2 * 4 * nv * NDET * NDET / (flt_ms * 1e-3) / 1e9 if flt_ms else None},
         "R_kernels_only": W / ((fp_ms + bp_ms) * 1e-3) / 1e9,
     }
 
@@ -179,7 +188,7 @@ def main():
     if not args.no_variants:
         for name, dt in (("bf16", torch.bfloat16), ("f16", torch.float16)):
             v16 = vol.to(dt)
-            w16, e16, f16ms, b16ms = timed(v16, dt, max(1, args.steps - 1), 1)
+            w16, e16, f16ms, b16ms, _ = timed(v16, dt, max(1, args.steps - 1), 1)
             variants[name] = {"ms_per_step": w16 * 1e3, "value": W / w16 / 1e9,
                               "fp_ms": f16ms, "bp_ms": b16ms,
                               "fp_gather_peak_Gtap_per_s": _native.gather_bench(
