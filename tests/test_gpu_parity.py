@@ -403,7 +403,9 @@ def test_abi_bad_device_raises(gpu):
 
 def test_config0_full_size_parity(gpu):
     """128^3 phantom, 180 views over 360 deg, 192^2 detector, pitch 0.75 voxel at the axis,
-    D = 500 voxels, step 0.5: FP and FDK backprojection each within 1e-4 of the FP64 oracle."""
+    D = 500 voxels, step 0.5: FP, the device FDK filter and FDK backprojection each within 1e-4
+    of the FP64 oracle, and the whole FDK chain (device filter + backprojection of the GPU's
+    projections) against the oracle's FP64 filter + backprojection of the same projections."""
     from fastcone_amd import ops, phantom
     from fastcone_amd.geometry import Geometry
     vol, _ = phantom.config_phantom(0, device=gpu)
@@ -416,11 +418,15 @@ def test_config0_full_size_parity(gpu):
     ref = oracle.project(prob, vol_np, h=0.5)
     e_fp = rel(p.cpu(), ref)
     q = ops.fdk_filter(p, g, 500.0, 0.75)
+    q_ref = oracle.fdk_filter(prob, p.cpu().numpy())
+    e_flt = rel(q.cpu(), q_ref)
     v = ops.backproject(q, g, mode=2, norm=torch.full((180,), 1 / 180, device=gpu)).cpu()
     ref_bp = oracle.backproject(prob, q.cpu().numpy(), (128, 128, 128), mode=2)
     e_bp = rel(v, ref_bp)
-    print(f"config 0: FP rel L2 {e_fp:.2e}, FDK BP rel L2 {e_bp:.2e}")
-    assert e_fp < 1e-4 and e_bp < 1e-4
+    e_chain = rel(v, oracle.backproject(prob, q_ref, (128, 128, 128), mode=2))
+    print(f"config 0: FP rel L2 {e_fp:.2e}, filter {e_flt:.2e}, FDK BP {e_bp:.2e}, "
+          f"filter + BP chain {e_chain:.2e}")
+    assert e_fp < 1e-4 and e_flt < 5e-5 and e_bp < 1e-4 and e_chain < 1e-4
     # FDK reconstructs the phantom (inside the field of view) to the usual FDK accuracy
     inner = (slice(40, 88),) * 3
     err = np.linalg.norm(v.numpy()[inner] - vol_np[inner]) / np.linalg.norm(vol_np[inner])
