@@ -8,12 +8,17 @@
 //    x row and all output channels; each lane feeds 8 consecutive k (one 16-byte load of 8
 //    channels of one tap). Bias and ReLU are fused in the epilogue. The input gradient is the
 //    same kernel on dY with the flipped, transposed weights  W'[ci][26 - tap][co] = W[co][tap][ci].
-//  * weight gradient: dW[co][k] = sum_voxel dY[voxel][co] P[voxel][k] with v_mfma_f32_16x16x4_f32,
-//    a block per (group of x rows, 16 k), reduced over its 4 waves in LDS and added to dW with
-//    float atomics (FP32 master gradient).
-//  * 1-channel layers (first layer Cin = 1, last layer Cout = 1): direct FP32 VALU kernels.
+//  * weight gradient: dW[co][k] = sum_voxel dY[voxel][co] P[voxel][k] with
+//    v_mfma_f32_16x16x32_bf16 fed the raw BF16 bits (products exact in FP32): a block per (group
+//    of x rows, 64 k), one 16-column slice of K per wave; each row's output gradient is staged
+//    once per block in LDS, transposed to channel-major in chunks of 256 voxels, so a lane's 8
+//    voxels of one channel are one 16-byte LDS read; the input is gathered per tap. Each block
+//    adds its partial dW to the FP32 master gradient with float atomics.
+//  * 1-channel layers (first layer Cin = 1, last layer Cout = 1): direct FP32 VALU kernels
+//    (forward, input gradient, weight gradient).
 //
-// Operand / result lane layouts of the MFMA forms used (checked by fc_mfma_layout_probe):
+// Operand / result lane layouts of the MFMA form used (checked by fc_mfma_layout_probe, which also
+// checks the 16x16x4 FP32 form):
 //   16x16x32 bf16: A lane l holds A[l % 16][8 (l / 16) + 0..7], B lane l holds
 //   B[8 (l / 16) + 0..7][l % 16]; 16x16x4 f32: A[l % 16][l / 16], B[l / 16][l % 16];
 //   result D lane l, register r holds D[4 (l / 16) + r][l % 16].
@@ -106,8 +111,9 @@ __global__ __launch_bounds__(256) void flip_transpose_weights(const uint16_t* __
 
 // ---- weight gradient -------------------------------------------------------------------------
 // grid.x: groups of kRows x-rows (n, z, y); grid.y: blocks of 64 k, one 16-column slice per wave.
-// The four waves read the same output-gradient values (served by L1 after the first wave), so each
-// is fetched from L2 once per 64 columns of K instead of once per 16. dW (FP32) is accumulated.
+// The row's output gradient is staged once per block in LDS (channel-major, 256-voxel chunks) and
+// read by all four waves, so it is fetched from HBM / L2 once per 64 columns of K. dW (FP32) is
+// accumulated.
 constexpr int kRows = 8;
 constexpr int kWgChunk = 256;            // voxels of a row staged at once
 constexpr int kWgPitch = kWgChunk + 8;    // LDS row pitch (elements): 16-byte aligned rows
