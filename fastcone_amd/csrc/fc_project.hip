@@ -120,14 +120,36 @@ struct FpArgs {
 
 // Tile order: groups of `group` consecutive views; inside a group the same pixel tile of all
 // its views is adjacent, so concurrently running blocks of one XCD gather the same voxels.
+// The groups are dealt to the eight XCDs' contiguous block ranges by residue: the range at
+// position x takes groups x, x + 8, x + 16, ..., so every XCD sees views from the whole scan.
+// Views whose rays run along x fill their LDS bricks more slowly than views along y, and an XCD
+// holding one contiguous eighth of a circular scan would finish its octant late or early.
 // Returns the view t and the tile's first column / row (j0, i0); the thread's pixel is
 // (j0 + (wave & 1) * 8 + (lane & 7), i0 + (wave >> 1) * 8 + (lane >> 3)).
 __device__ __forceinline__ void fp_tile_origin(int tiles_w, int tiles_per_view, int group,
                                                int nviews, int* t, int* j0, int* i0) {
   int lb = xcd_remap(blockIdx.x, gridDim.x);
   int per_group = tiles_per_view * group;
-  int g = lb / per_group;
-  int r = lb - g * per_group;
+  int p = lb / per_group;
+  int r = lb - p * per_group;
+  // Only the full groups are dealt (a partial last group keeps the last position, where the
+  // grid has its smaller block range): position p among the nfull -> range x of eight
+  // near-equal ranges, index i in it -> group x + 8 i.
+  const int nfull = nviews / group;
+  int g = p;
+  if (p < nfull) {
+    const int per = nfull >> 3, rem = nfull & 7;
+    int x, i;
+    if (p < rem * (per + 1)) {
+      x = p / (per + 1);
+      i = p - x * (per + 1);
+    } else {
+      const int pp = p - rem * (per + 1);
+      x = rem + pp / per;
+      i = pp - (x - rem) * per;
+    }
+    g = x + 8 * i;
+  }
   int gsize = min(group, nviews - g * group);
   int tile = r / gsize;
   int v = r - tile * gsize;
@@ -340,19 +362,28 @@ __device__ __forceinline__ int4 slab_box(const QuadGrid& G, const SlabAxes& X, i
   return make_int4(e1lo, e2lo, e1hi - e1lo + 1, e2hi - e2lo + 1);
 }
 
+// Plane pitch (floats) of an LDS brick of E1 rows of DP floats: DP E1 rounded up to 8 mod 16,
+// so consecutive planes start 8 or 24 of the 32 four-byte banks apart. A replay of the lanes'
+// addresses (8 x 8 rays per wave, 0.75-voxel spacing, the eight corner reads of each sample) on
+// 32 banks puts this among the lowest conflict counts of all plane offsets: 1.44 bank passes per
+// ideal pass against 1.97 unpadded.
+__host__ __device__ constexpr uint32_t brick_plane(uint32_t DP, uint32_t E1) {
+  return DP * E1 + ((8u - DP * E1) & 15u);
+}
+
 // Trilinear sample from an LDS brick laid out dominant axis fastest (pitch DP), then minor axis 1
-// (E1 rows), then minor axis 2 (plane of zbytes bytes). `base` is the brick address minus the
-// brick origin's offset, so the corner of a sample at brick-frame-relative coordinates
-// (qd, q1, q2) >= 0 is at index qd + DP (q1 + E1 q2). Truncation is floor for q >= 0; the 24-bit
+// (rows), then minor axis 2 (planes of PSu floats = zbytes bytes). `base` is the brick address
+// minus the brick origin's offset, so the corner of a sample at brick-frame-relative coordinates
+// (qd, q1, q2) >= 0 is at index qd + DP q1 + PSu q2. Truncation is floor for q >= 0; the 24-bit
 // multiply-adds keep the index math full rate.
 template <int DP>
-__device__ __forceinline__ float lds_trilinear(const char* base, uint32_t E1u, uint32_t zbytes,
+__device__ __forceinline__ float lds_trilinear(const char* base, uint32_t PSu, uint32_t zbytes,
                                                float qd, float q1, float q2) {
   const uint32_t id = (uint32_t)(int)qd, i1 = (uint32_t)(int)q1, i2 = (uint32_t)(int)q2;
   const float ad = __builtin_amdgcn_fractf(qd), a1f = __builtin_amdgcn_fractf(q1),
               a2f = __builtin_amdgcn_fractf(q2);
-  const uint32_t row = mad_u24(i2, E1u, i1);
-  const float* L0 = reinterpret_cast<const float*>(base + 4u * mad_u24(row, (uint32_t)DP, id));
+  const uint32_t idx = mad_u24(i2, PSu, mad_u24(i1, (uint32_t)DP, id));
+  const float* L0 = reinterpret_cast<const float*>(base + 4u * idx);
   const float* L1 = reinterpret_cast<const float*>(reinterpret_cast<const char*>(L0) + zbytes);
   float c00 = lerp(L0[0], L0[1], ad), c10 = lerp(L0[DP], L0[DP + 1], ad);
   float c01 = lerp(L1[0], L1[1], ad), c11 = lerp(L1[DP], L1[DP + 1], ad);
@@ -401,8 +432,8 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
     const int kout = min(max((int)ceilf((Pout - pd0) * inv_sd), 0), r.nk);
     if (E1 <= 0 || E2 <= 0) continue;  // the frustum misses the block here
     const int dlo = (int)lo - 1;
-    const int PS = DP * E1;
-    const int NE = PS * E2;
+    const uint32_t PS = brick_plane(DP, (uint32_t)E1), pad = PS - (uint32_t)(DP * E1);
+    const int NE = (int)PS * E2;
     if (NE > lds_cap) {
       for (int k = kin; k < kout; ++k) {
         float kf = (float)k;
@@ -434,21 +465,23 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
         const int nrow = E1 * E2;
 #pragma unroll 1
         for (int r0 = kRowsW * wave + (int)rr; r0 < nrow; r0 += 4 * kRowsB) {
-          uint32_t v[4];
+          uint32_t v[4], w[4];
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const int rw = r0 + q * kRowsB;
             v[q] = 0u;
+            w[q] = 0u;
             if (rw < nrow) {
               const uint32_t m2 = (uint32_t)(((float)rw + 0.5f) * invE1);
               const uint32_t m1 = (uint32_t)rw - mul_u24(m2, (uint32_t)E1);
               v[q] = load_bits<DT>(a.data, mad_u24(m1, X.gs1, mad_u24(m2, X.gs2, gd)));
+              w[q] = mad_u24(m2, pad, mad_u24((uint32_t)rw, (uint32_t)DP, d));
             }
           }
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const int rw = r0 + q * kRowsB;
-            if (rw < nrow) brick[d + DP * (uint32_t)rw] = bits_to_f32<DT>(v[q]);
+            if (rw < nrow) brick[w[q]] = bits_to_f32<DT>(v[q]);
           }
         }
       }
@@ -464,20 +497,20 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
 #pragma unroll
         for (int d = 0; d < DP; ++d)
           v[d] = load_elem<DT>(a.data, mad_u24(min((uint32_t)d, dmax), X.gsd, g));
+        float* col = brick + mad_u24(m2, pad, mul_u24((uint32_t)c, (uint32_t)DP));
 #pragma unroll
-        for (int d = 0; d < DP; ++d) brick[d + DP * c] = v[d];
+        for (int d = 0; d < DP; ++d) col[d] = v[d];
       }
     }
     __syncthreads();
     // Brick-relative LDS byte offset of a sample's corner (0,0,0). Sample coordinates are >= 0
     // in the padded frame (ray_segment guarantees it), so truncation is floor; the 24-bit
     // multiply-adds stay full rate.
-    const uint32_t E1u = (uint32_t)E1;
-    const uint32_t boff = (uint32_t)(dlo + DP * (B.x + E1 * B.y));
-    const uint32_t zbytes = 4u * (uint32_t)PS;  // one m2 plane
+    const uint32_t boff = (uint32_t)(dlo + DP * B.x) + PS * (uint32_t)B.y;
+    const uint32_t zbytes = 4u * PS;  // one m2 plane
     const char* base = reinterpret_cast<const char*>(brick) - 4u * boff;
     auto sample = [&](float kf) {
-      return lds_trilinear<DP>(base, E1u, zbytes, fmaf(kf, sd, pd0), fmaf(kf, s1, p10),
+      return lds_trilinear<DP>(base, PS, zbytes, fmaf(kf, sd, pd0), fmaf(kf, s1, p10),
                                fmaf(kf, s2, p20));
     };
     // two samples per iteration: both samples' LDS reads are in flight together
@@ -533,7 +566,8 @@ void fp_quad_kernel(FpArgs a, uint8_t* __restrict__ quad_mode, int mode) {
       const int4 B = kind < 4 ? slab_box<S>(G, X, 0, kQuad, (kind & 1) * 2, (kind >> 1) * 2, 0, m)
                               : slab_box<S>(G, X, kind - 4, 1, 0, 0, 2, m);
       if (B.z > 0 && B.w > 0) {
-        const unsigned long long ne = (unsigned long long)DP * (unsigned)B.z * (unsigned)B.w;
+        const unsigned long long ne =
+            (unsigned long long)brick_plane(DP, (uint32_t)B.z) * (unsigned)B.w;
         const int cap = kind < 4 ? min(a.lds_cap, kQuadCap) : a.lds_cap;
         const unsigned long long c = ne > (unsigned long long)cap ? 4ull * ne : ne;
         if (kind < 4) csh += c; else csi += c;
@@ -1031,6 +1065,7 @@ __global__ __launch_bounds__(256) void gather_bench_lds(FpArgs a, int E, int nsa
                                                         float* __restrict__ sink) {
   constexpr int DP = S + 3;
   __shared__ float brick[kLdsCap];
+  const uint32_t PS = brick_plane(DP, (uint32_t)E);
   const int NE = DP * E * E;
   const int nb = (a.nx - E) / E;
   const int b = blockIdx.x % (nb * nb);
@@ -1038,14 +1073,14 @@ __global__ __launch_bounds__(256) void gather_bench_lds(FpArgs a, int E, int nsa
                       (uint32_t)(2 + (b / nb) * E) * (uint32_t)a.PX * (uint32_t)a.PY + 2u;
   for (int e = threadIdx.x; e < NE; e += 256) {
     const int d = e % DP, rest = e / DP, m1 = rest % E, m2 = rest / E;
-    brick[e] = static_cast<const float*>(a.data)[g0 + d + (uint32_t)m1 * a.PX +
-                                                 (uint32_t)m2 * a.PX * a.PY];
+    brick[d + DP * m1 + PS * m2] = static_cast<const float*>(a.data)[g0 + d + (uint32_t)m1 * a.PX +
+                                                                     (uint32_t)m2 * a.PX * a.PY];
   }
   __syncthreads();
   uint32_t s = 0x9e3779b9u * (blockIdx.x * 256 + threadIdx.x + 1);
   const float sd = (float)(DP - 1) * (1.0f / 16777216.0f), se = (float)(E - 1) * (1.0f / 16777216.0f);
   const char* base = reinterpret_cast<const char*>(brick);
-  const uint32_t zbytes = 4u * DP * E;
+  const uint32_t zbytes = 4u * PS;
   float acc = 0.0f;
   for (int k = 0; k < nsamp; k += 2) {
     float q[6];
@@ -1054,8 +1089,8 @@ __global__ __launch_bounds__(256) void gather_bench_lds(FpArgs a, int E, int nsa
       s = s * 1664525u + 1013904223u;
       q[c] = (float)(s >> 8) * ((c % 3) == 0 ? sd : se);
     }
-    const float r0 = lds_trilinear<DP>(base, (uint32_t)E, zbytes, q[0], q[1], q[2]);
-    const float r1 = lds_trilinear<DP>(base, (uint32_t)E, zbytes, q[3], q[4], q[5]);
+    const float r0 = lds_trilinear<DP>(base, PS, zbytes, q[0], q[1], q[2]);
+    const float r1 = lds_trilinear<DP>(base, PS, zbytes, q[3], q[4], q[5]);
     acc += r0;
     acc += r1;
   }
@@ -1063,10 +1098,11 @@ __global__ __launch_bounds__(256) void gather_bench_lds(FpArgs a, int E, int nsa
 }
 
 // Ray-coherent LDS gather ceiling: the projector's trilinear LDS sample (same lds_trilinear,
-// positions p0 + k s by FMA) over one resident 11 x 16 x 16 brick, with the projector's lane
-// layout (8 x 8 rays per wave, 16 x 16 per block) and ray-like positions (0.75-voxel pixel
-// spacing, diverging directions, 16 samples per slab pass), U samples per iteration. The LDS
-// allocation is exactly the brick (11 KiB), so occupancy is limited by registers only. No refill,
+// positions p0 + k s by FMA) over one resident 11 x 16 x 16 brick (padded planes, as in the
+// projector), with the projector's lane layout (8 x 8 rays per wave, 16 x 16 per block) and
+// ray-like positions (0.75-voxel pixel spacing, diverging directions, 16 samples per slab pass),
+// U samples per iteration. The LDS
+// allocation is exactly the brick (11.5 KiB), so occupancy is limited by registers only. No refill,
 // no ray set-up and no divergence: it bounds the tap rate of LDS trilinear sampling along rays.
 // Bounds: qd <= 1 + 15 * 0.47 < 9, q1, q2 in [1.25 - 1.125, 1.25 + 11.25 + 1.125 + 0.25] =
 // [0.125, 13.875], so every corner index stays in [0, 9] x [0, 14] x [0, 14].
@@ -1074,15 +1110,16 @@ template <int S, int U>
 __global__ __launch_bounds__(256) void gather_bench_lds_ray(FpArgs a, int reps,
                                                             float* __restrict__ sink) {
   constexpr int DP = S + 3, E = 16, NE = DP * E * E;
-  __shared__ float brick[NE];
+  constexpr uint32_t PS = brick_plane(DP, E);
+  __shared__ float brick[PS * E];
   const int nb = (a.nx - E) / E;
   const int b = blockIdx.x % (nb * nb);
   const uint32_t g0 = (uint32_t)(2 + (b % nb) * E) * (uint32_t)a.PX +
                       (uint32_t)(2 + (b / nb) * E) * (uint32_t)a.PX * (uint32_t)a.PY + 2u;
   for (int e = threadIdx.x; e < NE; e += 256) {
     const int d = e % DP, rest = e / DP, m1 = rest % E, m2 = rest / E;
-    brick[e] = static_cast<const float*>(a.data)[g0 + d + (uint32_t)m1 * a.PX +
-                                                 (uint32_t)m2 * a.PX * a.PY];
+    brick[d + DP * m1 + PS * m2] = static_cast<const float*>(a.data)[g0 + d + (uint32_t)m1 * a.PX +
+                                                                     (uint32_t)m2 * a.PX * a.PY];
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1090,7 +1127,7 @@ __global__ __launch_bounds__(256) void gather_bench_lds_ray(FpArgs a, int reps,
   const float sd = 0.47f, s1 = 0.01f * (jj - 7.5f), s2 = 0.01f * (ii - 7.5f);
   const float pd0 = 1.0f;
   const char* base = reinterpret_cast<const char*>(brick);
-  const uint32_t zbytes = 4u * DP * E;
+  const uint32_t zbytes = 4u * PS;
   float acc = 0.0f;
 #pragma unroll 1
   for (int rep = 0; rep < reps; ++rep) {
@@ -1102,7 +1139,7 @@ __global__ __launch_bounds__(256) void gather_bench_lds_ray(FpArgs a, int reps,
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const float kf = (float)(k + u);
-        r[u] = lds_trilinear<DP>(base, (uint32_t)E, zbytes, fmaf(kf, sd, pd0),
+        r[u] = lds_trilinear<DP>(base, PS, zbytes, fmaf(kf, sd, pd0),
                                  fmaf(kf, s1, p10), fmaf(kf, s2, p20));
       }
 #pragma unroll
