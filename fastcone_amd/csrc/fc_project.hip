@@ -238,22 +238,24 @@ __global__ __launch_bounds__(256) void fp_kernel(FpArgs a) {
 //     sample are two base addresses with immediate offsets {0, 1, S+3, S+4};
 //   * a ray's samples are split between slabs by integer k ranges computed from the shared slab
 //     planes, so every sample is evaluated exactly once.
-// Work is dealt in "quads": one 16x16 pixel tile in four consecutive views, four blocks. Adjacent
-// views' rays cross nearly the same voxels, so in "shared" mode each block takes one 8x8 quarter
-// of the tile in all four views (one view per wave) and its bricks are the union of the four
-// views' frustum slices: one fill serves four views (fp_quad_kernel, 8 waves per SIMD, 18 KiB
-// bricks). When the views are far apart (sparse angular sampling) that union is wide, and the
-// quad is left to fp_lds_kernel, where each block takes the whole 16x16 tile of one view
-// (6 waves per SIMD, 24 KiB bricks). fp_quad_kernel decides per quad from the brick elements of
-// both layouts summed over every fourth slab (a brick over its budget counting 4x; the last quad
-// of a view count that is not a multiple of four, and a quad whose grid rays do not all advance
-// along the dominant axis, always go to fp_lds_kernel) and records the decision in a byte per
+// Work is dealt in "quads": one 16x16 pixel tile in four consecutive views. Adjacent views' rays
+// cross nearly the same voxels, so in "shared" mode each 512-thread block takes one 16x8 half of
+// the tile in all four views (one wave per view and 8x8 quarter) and its bricks are the union of
+// the four views' frustum slices over the half: one fill serves eight waves (fp_quad_kernel, two
+// blocks per quad, 8 waves per SIMD, bricks up to 32 KiB). A half shares its bricks between two
+// quarters, which costs 30 % fewer brick elements per ray sample than a brick per quarter. When
+// the views are far apart (sparse angular sampling) the union is wide, and the quad is left to
+// fp_lds_kernel, where each 256-thread block takes the whole 16x16 tile of one view (6 waves per
+// SIMD, 24 KiB bricks). fp_quad_kernel decides per quad from the brick elements of both layouts
+// summed over every fourth slab (a brick over its budget counting 4x; a quad whose grid rays do
+// not all advance along the dominant axis always goes to fp_lds_kernel) and records the decision
+// in a byte per
 // quad that fp_lds_kernel, launched next on the same stream, reads.
 // Bricks larger than the LDS budget (or tiles whose rays do not all advance along the dominant
 // axis) fall back to gathering from the staged volume in global memory, with the same arithmetic.
 
 constexpr int kQuad = 4;          // views per quad
-constexpr int kQuadCap = 4608;    // fp_quad_kernel's LDS brick budget, floats (18 KiB)
+constexpr int kHalfCap = 8192;    // fp_quad_kernel's LDS brick budget, floats (32 KiB)
 constexpr int kSlabTab = 64;      // slab boxes computed per table refill
 constexpr uint8_t kQuadShared = 1;
 
@@ -261,8 +263,8 @@ constexpr uint8_t kQuadShared = 1;
 // pixels, columns j0 + {0, 7, 8, 15} and rows i0 + {0, 7, 8, 15} (clamped to the detector): their
 // minor / dominant direction ratios (r1, r2), plus the view's source in the padded frame and its
 // interpolation margin. A sub-tile's four corner rays are the grid points (cj, ci) with
-// cj in {cj0, cj0 + 1 + cw}, ci in {ci0, ci0 + 1 + cw}: cw = 0 for an 8x8 quarter (cj0, ci0 in
-// {0, 2}), cw = 2 for the whole tile (cj0 = ci0 = 0).
+// cj in {cj0, cj0 + 1 + cwj}, ci in {ci0, ci0 + 1 + cwi}: cwj = cwi = 2 for the whole tile
+// (cj0 = ci0 = 0), cwj = 2, cwi = 0 for a 16x8 half (cj0 = 0, ci0 in {0, 2}).
 struct QuadGrid {
   float r1[kQuad][16], r2[kQuad][16];
   float src[kQuad][4];  // dominant, minor 1, minor 2, margin
@@ -333,7 +335,7 @@ __device__ __forceinline__ bool grid_ray(QuadGrid& G, const FpArgs& a, const Sla
 // (e1lo, e2lo, E1, E2).
 template <int S>
 __device__ __forceinline__ int4 slab_box(const QuadGrid& G, const SlabAxes& X, int v0, int nv,
-                                         int cj0, int ci0, int cw, int m) {
+                                         int cj0, int ci0, int cwj, int cwi, int m) {
   const int b = X.dir > 0.0f ? m : X.nslabs - 1 - m;
   const float lo = 1.0f + (float)(b * S), hi = lo + (float)S;
   float mn1 = 3e38f, mx1 = -3e38f, mn2 = 3e38f, mx2 = -3e38f, mg = 0.0f;
@@ -343,7 +345,7 @@ __device__ __forceinline__ int4 slab_box(const QuadGrid& G, const SlabAxes& X, i
     mg = fmaxf(mg, G.src[v][3]);
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      const int p = (ci0 + (c >> 1) * (1 + cw)) * 4 + cj0 + (c & 1) * (1 + cw);
+      const int p = (ci0 + (c >> 1) * (1 + cwi)) * 4 + cj0 + (c & 1) * (1 + cwj);
       const float q1 = G.r1[v][p], q2 = G.r2[v][p];
       float u1 = fmaf(tlo, q1, s1), w1 = fmaf(thi, q1, s1);
       float u2 = fmaf(tlo, q2, s2), w2 = fmaf(thi, q2, s2);
@@ -392,13 +394,15 @@ __device__ __forceinline__ float lds_trilinear(const char* base, uint32_t PSu, u
 }
 
 // Sum of this thread's ray samples, slab by slab through LDS bricks whose boxes come from G
-// (views v0 .. v0 + nv - 1, sub-tile (cj0, ci0, cw)), or from global memory when `fallback`.
-template <int DT, int S>
+// (views v0 .. v0 + nv - 1, sub-tile (cj0, ci0, cwj, cwi)), or from global memory when
+// `fallback`. NT threads per block.
+template <int DT, int S, int NT>
 __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
                                        const QuadGrid& G, const SlabAxes& X, int v0, int nv,
-                                       int cj0, int ci0, int cw, const RaySeg& r, bool fallback,
-                                       int lds_cap) {
+                                       int cj0, int ci0, int cwj, int cwi, const RaySeg& r,
+                                       bool fallback, int lds_cap) {
   constexpr int DP = S + 3;  // dominant-axis pitch of the LDS brick
+  constexpr int NW = NT / 64;  // waves per block
   float acc = 0.0f;
   if (fallback) {
     for (int k = 0; k < r.nk; ++k) {
@@ -420,7 +424,7 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
       __syncthreads();  // the previous table is no longer read
       const int mm = m + (int)threadIdx.x;
       if (threadIdx.x < kSlabTab && mm < X.nslabs)
-        tab[threadIdx.x] = slab_box<S>(G, X, v0, nv, cj0, ci0, cw, mm);
+        tab[threadIdx.x] = slab_box<S>(G, X, v0, nv, cj0, ci0, cwj, cwi, mm);
       __syncthreads();
     }
     const int4 B = tab[m % kSlabTab];
@@ -455,9 +459,9 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
     const uint32_t dmax = (uint32_t)(X.PD - 1 - dlo);  // >= 2: dlo = b S <= PD - 3
     if (X.gsd == 1u) {
       // lanes 0..54 of each wave: d = lane % DP (dominant offset) and row phase lane / DP; the
-      // block's 4 waves cover 20 consecutive rows per step
+      // block's NW waves cover 5 NW consecutive rows per step
       const float invE1 = 1.0f / (float)E1;
-      constexpr int kLanes = (64 / DP) * DP, kRowsW = 64 / DP, kRowsB = 4 * kRowsW;
+      constexpr int kLanes = (64 / DP) * DP, kRowsW = 64 / DP, kRowsB = NW * kRowsW;
       if (lane < kLanes) {
         const uint32_t rr = (uint32_t)(((float)lane + 0.5f) * (1.0f / (float)DP));
         const uint32_t d = (uint32_t)lane - mul_u24(rr, (uint32_t)DP);
@@ -489,7 +493,7 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
       const int ncol = E1 * E2;
       const float invE1 = 1.0f / (float)E1;
 #pragma unroll 1
-      for (int c = threadIdx.x; c < ncol; c += 256) {
+      for (int c = threadIdx.x; c < ncol; c += NT) {
         const uint32_t m2 = (uint32_t)(((float)c + 0.5f) * invE1);  // exact: c < 2^10
         const uint32_t m1 = (uint32_t)c - mul_u24(m2, (uint32_t)E1);
         const uint32_t g = mad_u24(m1, X.gs1, mad_u24(m2, X.gs2, gbase));
@@ -526,22 +530,26 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
   return acc;
 }
 
-// Shared-mode quads. Blocks (quad, quarter) decide the mode and record it; single-mode quads are
-// left to fp_lds_kernel. `mode` = 0: cost decision, 1: always shared.
+// Shared-mode quads. Blocks (quad, half) decide the mode and record it; single-mode quads are
+// left to fp_lds_kernel. `mode` = 0: cost decision, 1: always shared. Launched with half as many
+// blocks as fp_lds_kernel: block b covers "virtual view" 2 q + h (quad q, half h), so the tile
+// order of fp_tile_origin (groups, XCD dealing) applies with half the group size.
 template <int DT, int S>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8, 8)))
 void fp_quad_kernel(FpArgs a, uint8_t* __restrict__ quad_mode, int mode) {
   constexpr int DP = S + 3;
-  __shared__ float brick[kQuadCap];
+  __shared__ float brick[kHalfCap];
   __shared__ int4 tab[kSlabTab];
   __shared__ QuadGrid G;
   __shared__ unsigned long long cost[2];
   const int nvq = (a.nviews + kQuad - 1) / kQuad * kQuad;
-  int tg, j0, i0;
-  fp_tile_origin(a.tiles_w, a.tiles_per_view, a.group, nvq, &tg, &j0, &i0);
-  const int sub = tg % kQuad;
-  const int t0 = tg - sub;
+  int tv, j0, i0;
+  fp_tile_origin(a.tiles_w, a.tiles_per_view, a.group / 2, nvq / 2, &tv, &j0, &i0);
+  const int half = tv & 1;
+  const int t0 = (tv >> 1) * kQuad;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // the budget: kHalfCap unless lowered below fp_lds_kernel's (tests of the global fallback)
+  const int cap_shared = a.lds_cap >= kLdsCap ? kHalfCap : a.lds_cap;
   const SlabAxes X =
       slab_axes<S>(a, a.views + (size_t)min(t0, a.nviews - 1) * FC_VIEW_DOUBLES, j0, i0);
   // the quad's 4x4 ray grid in each view, one thread per (view, point)
@@ -554,23 +562,24 @@ void fp_quad_kernel(FpArgs a, uint8_t* __restrict__ quad_mode, int mode) {
   }
   if (threadIdx.x < 2) cost[threadIdx.x] = 0ull;
   const bool quad_bad = __syncthreads_or(bad);
-  // items (kind, slab), kind 0-3: the quarters over all four views, 4-7: each view's whole
+  // items (kind, slab), kind 0-1: the two halves over all four views, 2-5: each view's whole
   // tile, over every fourth slab (brick sizes change slowly from slab to slab); consecutive
   // lanes take consecutive slabs of one kind; a brick over its budget counts 4x. Sums stay in
   // registers and are reduced across the wave, so each wave adds once to each LDS total.
   if (!quad_bad && mode == 0) {
     unsigned long long csh = 0ull, csi = 0ull;
     const int nsub = (X.nslabs + 3) / 4;
-    for (int w = threadIdx.x; w < 8 * nsub; w += 256) {
+    for (int w = threadIdx.x; w < 6 * nsub; w += 512) {
       const int kind = w / nsub, m = 4 * (w - kind * nsub);
-      const int4 B = kind < 4 ? slab_box<S>(G, X, 0, kQuad, (kind & 1) * 2, (kind >> 1) * 2, 0, m)
-                              : slab_box<S>(G, X, kind - 4, 1, 0, 0, 2, m);
+      if (kind >= 2 && t0 + kind - 2 >= a.nviews) continue;  // a missing view of the last quad
+      const int4 B = kind < 2 ? slab_box<S>(G, X, 0, kQuad, 0, kind * 2, 2, 0, m)
+                              : slab_box<S>(G, X, kind - 2, 1, 0, 0, 2, 2, m);
       if (B.z > 0 && B.w > 0) {
         const unsigned long long ne =
             (unsigned long long)brick_plane(DP, (uint32_t)B.z) * (unsigned)B.w;
-        const int cap = kind < 4 ? min(a.lds_cap, kQuadCap) : a.lds_cap;
+        const int cap = kind < 2 ? cap_shared : a.lds_cap;
         const unsigned long long c = ne > (unsigned long long)cap ? 4ull * ne : ne;
-        if (kind < 4) csh += c; else csi += c;
+        if (kind < 2) csh += c; else csi += c;
       }
     }
 #pragma unroll
@@ -585,16 +594,20 @@ void fp_quad_kernel(FpArgs a, uint8_t* __restrict__ quad_mode, int mode) {
   }
   __syncthreads();
   // A quad whose grid rays do not all advance along the dominant axis goes to fp_lds_kernel,
-  // which checks each view's tile on its own, and so does the last quad when the view count is
-  // not a multiple of four (shared mode would leave the waves of the missing views idle).
-  const bool shared = mode == 1 || (!quad_bad && t0 + kQuad <= a.nviews && cost[0] <= cost[1]);
+  // which checks each view's tile on its own. The last quad of a view count that is not a
+  // multiple of four is decided like the others (its missing views' waves idle in shared mode;
+  // left to fp_lds_kernel, whose block order keeps one view group on one XCD, its tiles would
+  // run on one eighth of the GPU after this kernel). Both halves' blocks evaluate the same
+  // decision from the same inputs.
+  const bool shared = mode == 1 || (!quad_bad && cost[0] <= cost[1]);
   const int tile = (i0 / kTile) * a.tiles_w + j0 / kTile;
-  if (sub == 0 && threadIdx.x == 0)
+  if (half == 0 && threadIdx.x == 0)
     quad_mode[(size_t)(t0 / kQuad) * a.tiles_per_view + tile] = shared ? kQuadShared : 0;
   if (!shared) return;
-  const int t = t0 + wave;
-  const int j = j0 + (sub & 1) * 8 + (lane & 7);
-  const int i = i0 + (sub >> 1) * 8 + (lane >> 3);
+  // wave: view t0 + (wave & 3), quarter (wave >> 2) of the half (left / right 8 columns)
+  const int t = t0 + (wave & 3);
+  const int j = j0 + (wave >> 2) * 8 + (lane & 7);
+  const int i = i0 + half * 8 + (lane >> 3);
   const bool pix = (t < a.nviews) && (j < a.nw) && (i < a.nh);
   RaySeg r;
   r.nk = 0;
@@ -605,8 +618,8 @@ void fp_quad_kernel(FpArgs a, uint8_t* __restrict__ quad_mode, int mode) {
   const float sdir = X.ax == 0 ? r.sx : (X.ax == 1 ? r.sy : r.sz);
   // every active ray must advance along the dominant axis in the same direction
   const bool fallback = __syncthreads_or(quad_bad || (r.nk > 0 && !(sdir * X.dir > 0.2f * (float)a.h)));
-  const float acc = march<DT, S>(a, brick, tab, G, X, 0, kQuad, (sub & 1) * 2, (sub >> 1) * 2, 0,
-                                 r, fallback, min(a.lds_cap, kQuadCap));
+  const float acc = march<DT, S, 512>(a, brick, tab, G, X, 0, kQuad, 0, half * 2, 2, 0, r,
+                                      fallback, cap_shared);
   if (pix) {
     float scale = (a.norm ? a.norm[t] : 1.0f) * (float)a.h;
     size_t idx = ((size_t)t * a.nh + i) * a.nw + j;
@@ -650,7 +663,8 @@ void fp_lds_kernel(FpArgs a, const uint8_t* __restrict__ quad_mode) {
   const float sdir = X.ax == 0 ? r.sx : (X.ax == 1 ? r.sy : r.sz);
   // every active ray must advance along the dominant axis in the same direction
   const bool fallback = __syncthreads_or(bad || (r.nk > 0 && !(sdir * X.dir > 0.2f * (float)a.h)));
-  const float acc = march<DT, S>(a, brick, tab, G, X, 0, 1, 0, 0, 2, r, fallback, a.lds_cap);
+  const float acc = march<DT, S, 256>(a, brick, tab, G, X, 0, 1, 0, 0, 2, 2, r, fallback,
+                                      a.lds_cap);
   if (pix) {
     float scale = (a.norm ? a.norm[t] : 1.0f) * (float)a.h;
     size_t idx = ((size_t)t * a.nh + i) * a.nw + j;
@@ -1241,9 +1255,11 @@ int launch_project(int device, const void* vol, int dt, const int vs[3], const d
       fp_kernel<FC_F32, FP_PLAIN><<<nb, 256, 0, stream>>>(a);
     } else {
       if (qmode >= 0) {
-        if (dt == FC_F32) fp_quad_kernel<FC_F32, 8><<<nb, 256, 0, stream>>>(a, quad_mode, qmode);
-        else if (dt == FC_BF16) fp_quad_kernel<FC_BF16, 8><<<nb, 256, 0, stream>>>(a, quad_mode, qmode);
-        else fp_quad_kernel<FC_F16, 8><<<nb, 256, 0, stream>>>(a, quad_mode, qmode);
+        // two 512-thread blocks per quad and tile (nb = tiles x nvq is a multiple of four)
+        const int nh = nb / 2;
+        if (dt == FC_F32) fp_quad_kernel<FC_F32, 8><<<nh, 512, 0, stream>>>(a, quad_mode, qmode);
+        else if (dt == FC_BF16) fp_quad_kernel<FC_BF16, 8><<<nh, 512, 0, stream>>>(a, quad_mode, qmode);
+        else fp_quad_kernel<FC_F16, 8><<<nh, 512, 0, stream>>>(a, quad_mode, qmode);
       }
       if (qmode <= 0) {
         if (dt == FC_F32) fp_lds_kernel<FC_F32, 8><<<nb, 256, 0, stream>>>(a, quad_mode);
