@@ -31,6 +31,7 @@ constexpr int kTile = 16;     // 16x16 pixel tile per 256-thread block, 8x8 per 
 constexpr int kLdsCap = 6144;  // fp_lds_kernel's LDS brick budget, floats per block (24 KiB)
 
 typedef float f32x2_a4 __attribute__((ext_vector_type(2), aligned(4)));
+typedef float f32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
 
 enum FpLayout { FP_OCTET = 0, FP_PLAIN = 1, FP_LDS = 2 };
 
@@ -457,7 +458,33 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
     const uint32_t gbase = mad_u24((uint32_t)dlo, X.gsd,
                                    mad_u24((uint32_t)B.x, X.gs1, (uint32_t)B.y * X.gs2));
     const uint32_t dmax = (uint32_t)(X.PD - 1 - dlo);  // >= 2: dlo = b S <= PD - 3
-    if (X.gsd == 1u) {
+    if (DT == FC_F32 && X.gsd == 1u && dmax >= (uint32_t)(DP - 1)) {
+      // FP32, dominant axis x, every row whole: a row's DP elements are kSeg four-element
+      // segments starting at 4 s (the last at DP - 4, overlapping its neighbour: the shared
+      // elements are written twice with the same value), one 16-byte load per lane. Lanes
+      // 0 .. 3 kRowsW - 1 of each wave: segment lane % kSeg of row phase lane / kSeg.
+      constexpr int kSeg = (DP + 3) / 4, kRowsW = 64 / kSeg, kRowsB = NW * kRowsW;
+      const float invE1 = 1.0f / (float)E1;
+      if (lane < kSeg * kRowsW) {
+        const uint32_t rr = (uint32_t)(((float)lane + 0.5f) * (1.0f / (float)kSeg));
+        const uint32_t sg = (uint32_t)lane - mul_u24(rr, (uint32_t)kSeg);
+        const uint32_t d0 = min(4u * sg, (uint32_t)(DP - 4));
+        const float* src = static_cast<const float*>(a.data) + (gbase + d0);
+        const int nrow = E1 * E2;
+#pragma unroll 1
+        for (int rw = kRowsW * wave + (int)rr; rw < nrow; rw += kRowsB) {
+          const uint32_t m2 = (uint32_t)(((float)rw + 0.5f) * invE1);
+          const uint32_t m1 = (uint32_t)rw - mul_u24(m2, (uint32_t)E1);
+          const f32x4_a4 v =
+              *reinterpret_cast<const f32x4_a4*>(src + mad_u24(m1, X.gs1, m2 * X.gs2));
+          float* dst = brick + mad_u24(m2, pad, mad_u24((uint32_t)rw, (uint32_t)DP, d0));
+          dst[0] = v.x;
+          dst[1] = v.y;
+          dst[2] = v.z;
+          dst[3] = v.w;
+        }
+      }
+    } else if (X.gsd == 1u) {
       // lanes 0..54 of each wave: d = lane % DP (dominant offset) and row phase lane / DP; the
       // block's NW waves cover 5 NW consecutive rows per step
       const float invE1 = 1.0f / (float)E1;
