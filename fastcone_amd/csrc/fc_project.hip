@@ -544,15 +544,16 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
       return lds_trilinear<DP>(base, PS, zbytes, fmaf(kf, sd, pd0), fmaf(kf, s1, p10),
                                fmaf(kf, s2, p20));
     };
-    // two samples per iteration: both samples' LDS reads are in flight together
-    float kf = (float)kin;
-    int k = kin;
-    for (; k + 1 < kout; k += 2, kf += 2.0f) {
-      const float q0 = sample(kf), q1 = sample(kf + 1.0f);
+    // two samples per iteration: both samples' LDS reads are in flight together; the loop
+    // counts in FP32 (exact: k < 2^24), so one add and one compare serve both samples
+    const float kend = (float)kout;
+    float kf = (float)kin, kf1 = kf + 1.0f;
+    for (; kf1 < kend; kf += 2.0f, kf1 += 2.0f) {
+      const float q0 = sample(kf), q1 = sample(kf1);
       acc += q0;
       acc += q1;
     }
-    if (k < kout) acc += sample(kf);
+    if (kf < kend) acc += sample(kf);
   }
   return acc;
 }
