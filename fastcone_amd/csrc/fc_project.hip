@@ -428,7 +428,11 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
         tab[threadIdx.x] = slab_box<S>(G, X, v0, nv, cj0, ci0, cwj, cwi, mm);
       __syncthreads();
     }
-    const int4 B = tab[m % kSlabTab];
+    // the box is block-uniform: keep it (and everything derived from it) in scalar registers
+    const int4 Bl = tab[m % kSlabTab];
+    const int4 B =
+        make_int4(__builtin_amdgcn_readfirstlane(Bl.x), __builtin_amdgcn_readfirstlane(Bl.y),
+                  __builtin_amdgcn_readfirstlane(Bl.z), __builtin_amdgcn_readfirstlane(Bl.w));
     const int E1 = B.z, E2 = B.w;
     const int b = X.dir > 0.0f ? m : X.nslabs - 1 - m;
     const float lo = 1.0f + (float)(b * S), hi = lo + (float)S;
@@ -455,8 +459,9 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
     // being x so a wave's loads stay coalesced, and the DP elements along the dominant axis
     // unrolled (LDS index d + DP c, pitch DP odd so the writes are bank-conflict free). Planes
     // past the padded block (last slab) read its last plane and are never sampled.
-    const uint32_t gbase = mad_u24((uint32_t)dlo, X.gsd,
-                                   mad_u24((uint32_t)B.x, X.gs1, (uint32_t)B.y * X.gs2));
+    // block-uniform: plain arithmetic keeps it on the scalar unit
+    const uint32_t gbase =
+        (uint32_t)dlo * X.gsd + (uint32_t)B.x * X.gs1 + (uint32_t)B.y * X.gs2;
     const uint32_t dmax = (uint32_t)(X.PD - 1 - dlo);  // >= 2: dlo = b S <= PD - 3
     if (DT == FC_F32 && X.gsd == 1u && dmax >= (uint32_t)(DP - 1)) {
       // FP32, dominant axis x, every row whole: a row's DP elements are kSeg four-element
@@ -464,7 +469,9 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
       // elements are written twice with the same value), one 16-byte load per lane. Lanes
       // 0 .. 3 kRowsW - 1 of each wave: segment lane % kSeg of row phase lane / kSeg.
       constexpr int kSeg = (DP + 3) / 4, kRowsW = 64 / kSeg, kRowsB = NW * kRowsW;
-      const float invE1 = 1.0f / (float)E1;
+      // floor((r + 0.5) / E1) is exact for r < 2^13 with a 1-ulp reciprocal: (r + 0.5) / E1 lies
+      // at least 0.5 / E1 from an integer, and the two roundings move it by less than 2^-10 / E1
+      const float invE1 = __builtin_amdgcn_rcpf((float)E1);
       if (lane < kSeg * kRowsW) {
         const uint32_t rr = (uint32_t)(((float)lane + 0.5f) * (1.0f / (float)kSeg));
         const uint32_t sg = (uint32_t)lane - mul_u24(rr, (uint32_t)kSeg);
@@ -487,7 +494,7 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
     } else if (X.gsd == 1u) {
       // lanes 0..54 of each wave: d = lane % DP (dominant offset) and row phase lane / DP; the
       // block's NW waves cover 5 NW consecutive rows per step
-      const float invE1 = 1.0f / (float)E1;
+      const float invE1 = __builtin_amdgcn_rcpf((float)E1);  // exact as above
       constexpr int kLanes = (64 / DP) * DP, kRowsW = 64 / DP, kRowsB = NW * kRowsW;
       if (lane < kLanes) {
         const uint32_t rr = (uint32_t)(((float)lane + 0.5f) * (1.0f / (float)DP));
@@ -518,7 +525,7 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
       }
     } else {
       const int ncol = E1 * E2;
-      const float invE1 = 1.0f / (float)E1;
+      const float invE1 = __builtin_amdgcn_rcpf((float)E1);  // exact as above
 #pragma unroll 1
       for (int c = threadIdx.x; c < ncol; c += NT) {
         const uint32_t m2 = (uint32_t)(((float)c + 0.5f) * invE1);  // exact: c < 2^10
