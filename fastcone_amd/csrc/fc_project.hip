@@ -420,6 +420,13 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
               s2 = pick(X.a2, r.sx, r.sy, r.sz);
   const float inv_sd = 1.0f / sd;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int kprev;  // first k of the next slab: k >= its entry plane
+  {
+    const int b0 = X.dir > 0.0f ? 0 : X.nslabs - 1;
+    const float lo0 = 1.0f + (float)(b0 * S);
+    const float Pin0 = X.dir > 0.0f ? lo0 : lo0 + (float)S;
+    kprev = min(max((int)ceilf((Pin0 - pd0) * inv_sd), 0), r.nk);
+  }
   for (int m = 0; m < X.nslabs; ++m) {
     if (m % kSlabTab == 0) {
       __syncthreads();  // the previous table is no longer read
@@ -436,9 +443,12 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
     const int E1 = B.z, E2 = B.w;
     const int b = X.dir > 0.0f ? m : X.nslabs - 1 - m;
     const float lo = 1.0f + (float)(b * S), hi = lo + (float)S;
-    const float Pin = X.dir > 0.0f ? lo : hi, Pout = X.dir > 0.0f ? hi : lo;
-    const int kin = min(max((int)ceilf((Pin - pd0) * inv_sd), 0), r.nk);
+    const float Pout = X.dir > 0.0f ? hi : lo;
+    // slab m's entry plane is slab m - 1's exit plane (the same float), so its first k is the
+    // previous slab's end
+    const int kin = kprev;
     const int kout = min(max((int)ceilf((Pout - pd0) * inv_sd), 0), r.nk);
+    kprev = kout;
     if (E1 <= 0 || E2 <= 0) continue;  // the frustum misses the block here
     const int dlo = (int)lo - 1;
     const uint32_t PS = brick_plane(DP, (uint32_t)E1), pad = PS - (uint32_t)(DP * E1);
@@ -483,7 +493,7 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
           const uint32_t m2 = (uint32_t)(((float)rw + 0.5f) * invE1);
           const uint32_t m1 = (uint32_t)rw - mul_u24(m2, (uint32_t)E1);
           const f32x4_a4 v =
-              *reinterpret_cast<const f32x4_a4*>(src + mad_u24(m1, X.gs1, m2 * X.gs2));
+              *reinterpret_cast<const f32x4_a4*>(src + mad_u24(m1, X.gs1, mul_u24(m2, X.gs2)));
           float* dst = brick + mad_u24(m2, pad, mad_u24((uint32_t)rw, (uint32_t)DP, d0));
           dst[0] = v.x;
           dst[1] = v.y;
