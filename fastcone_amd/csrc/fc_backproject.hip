@@ -71,7 +71,11 @@ __global__ __launch_bounds__(256) void stage_quads(const void* __restrict__ proj
 
 constexpr int kTileW = 64;                 // widest staged window (one pixel per lane)
 constexpr int kPre = 12;                   // rows per wave and view: window height <= 4 kPre
-constexpr int kTileCap = kTileW * 4 * kPre;  // floats per LDS buffer (12 KiB)
+// Row pitch of a staged window (floats): fixed, so a bilinear sample's two row pairs are one
+// base address with immediate offsets {0, 1, kTileP, kTileP + 1}; odd, so rows do not start in
+// the same bank.
+constexpr uint32_t kTileP = kTileW + 1;
+constexpr int kTileCap = kTileP * 4 * kPre;  // floats per LDS buffer (12.2 KiB)
 constexpr uint32_t kWinSkip = 0u;          // window word y: the cube misses the detector
 constexpr uint32_t kWinGlobal = 0xffffu;   // window word y low half: gather from global memory
 
@@ -164,14 +168,12 @@ __device__ __forceinline__ float clampf(float v, float lo, float hi) {
   return fminf(fmaxf(v, lo), hi);
 }
 
-// Bilinear sample at tile coordinates (u, v) >= 0 of an LDS window with row pitch Wu.
-__device__ __forceinline__ float tile_bilinear(const float* T, uint32_t Wu, uint32_t iu, float au,
-                                               float v) {
+// Bilinear sample at tile coordinates (u, v) >= 0 of an LDS window (row pitch kTileP).
+__device__ __forceinline__ float tile_bilinear(const float* T, uint32_t iu, float au, float v) {
   const uint32_t iv = (uint32_t)(int)v;
   const float av = __builtin_amdgcn_fractf(v);
-  const float* p = T + mad_u24(iv, Wu, iu);
-  const float* q = p + Wu;
-  return lerp(lerp(p[0], p[1], au), lerp(q[0], q[1], au), av);
+  const float* p = T + mad_u24(iv, kTileP, iu);
+  return lerp(lerp(p[0], p[1], au), lerp(p[kTileP], p[kTileP + 1], au), av);
 }
 
 template <int DT, int MODE>
@@ -182,6 +184,9 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
   const int y = blockIdx.y * kBXY + (wave >> 1) * 8 + (lane >> 3);
   const int z0 = blockIdx.z * kZR;
   const double X = (double)x + a.ox, Y = (double)y + a.oy, Z0 = (double)z0 + a.oz;
+  // the whole 16 x 16 x 16 cube lies in the volume (no clamping of tile coordinates needed)
+  const bool inner = (int)(blockIdx.x + 1) * kBXY <= a.nx &&
+                     (int)(blockIdx.y + 1) * kBXY <= a.ny && z0 + kZR <= a.nz;
   const uint2* W = a.win + ((size_t)(blockIdx.z * a.nby + blockIdx.y) * a.nbx + blockIdx.x) *
                                (size_t)a.nviews;
   float acc[kZR];
@@ -213,11 +218,11 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
   auto commit = [&](float* T, uint2 w) {
     const uint32_t Wu = w.y & 0xffffu, Wv = w.y >> 16;
     if (Wu == kWinSkip || Wu == kWinGlobal) return;
-    float* Tw = T + (uint32_t)wv * Wu + (uint32_t)lane;
+    float* Tw = T + (uint32_t)wv * kTileP + (uint32_t)lane;
 #pragma unroll
     for (int q = 0; q < kPre; ++q) {
       if ((uint32_t)(wv + 4 * q) < Wv && (uint32_t)lane < Wu)
-        Tw[4u * Wu * (uint32_t)q] = bits_to_f32<DT>(pre[q]);
+        Tw[4u * kTileP * (uint32_t)q] = bits_to_f32<DT>(pre[q]);
     }
   };
 
@@ -276,10 +281,17 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
       if constexpr (MODE == FC_BP_KATSEVICH) wt = s * (float)rc;
       const uint32_t iu = (uint32_t)(int)u;
       const float au = __builtin_amdgcn_fractf(u);
+      if (inner) {
+        // every voxel of the cube is in the volume: its taps lie >= 1 pixel inside the window
 #pragma unroll
-      for (int k = 0; k < kZR; ++k) {
-        const float v = clampf(fmaf((float)k, dv, vb), 0.0f, vmax);
-        acc[k] = fmaf(wt, tile_bilinear(T, Wu, iu, au, v), acc[k]);
+        for (int k = 0; k < kZR; ++k)
+          acc[k] = fmaf(wt, tile_bilinear(T, iu, au, fmaf((float)k, dv, vb)), acc[k]);
+      } else {
+#pragma unroll
+        for (int k = 0; k < kZR; ++k) {
+          const float v = clampf(fmaf((float)k, dv, vb), 0.0f, vmax);
+          acc[k] = fmaf(wt, tile_bilinear(T, iu, au, v), acc[k]);
+        }
       }
     } else {
       const float af = (float)A, bf = (float)B, cf = (float)C;
@@ -293,7 +305,7 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
         float wt = s;
         if constexpr (MODE == FC_BP_FDK) wt = s * rc * rc;
         if constexpr (MODE == FC_BP_KATSEVICH) wt = s * rc;
-        acc[k] = fmaf(wt, tile_bilinear(T, Wu, (uint32_t)(int)u, __builtin_amdgcn_fractf(u), v),
+        acc[k] = fmaf(wt, tile_bilinear(T, (uint32_t)(int)u, __builtin_amdgcn_fractf(u), v),
                       acc[k]);
       }
     }
@@ -313,7 +325,8 @@ __global__ __launch_bounds__(256) void gather_bench_bp(const float* __restrict__
                                                        int nsamp, float* __restrict__ sink) {
   __shared__ float T[kTileCap];
   const float* g = src + (size_t)(blockIdx.x % 64) * 48 * pitch;
-  for (int e = threadIdx.x; e < kTileCap; e += 256) T[e] = g[(e / kTileW) * pitch + e % kTileW];
+  for (int e = threadIdx.x; e < kTileCap; e += 256)
+    T[e] = g[(e / kTileP) * pitch + min(e % kTileP, (uint32_t)kTileW - 1u)];
   __syncthreads();
   uint32_t s = 0x9e3779b9u * (blockIdx.x * 256 + threadIdx.x + 1);
   const float su = 62.0f / 16777216.0f, sv = 46.0f / 16777216.0f;
@@ -323,24 +336,24 @@ __global__ __launch_bounds__(256) void gather_bench_bp(const float* __restrict__
     const float u = (float)(s >> 8) * su;
     s = s * 1664525u + 1013904223u;
     const float v = (float)(s >> 8) * sv;
-    acc = fmaf(1.0001f, tile_bilinear(T, kTileW, (uint32_t)(int)u, __builtin_amdgcn_fractf(u), v),
-               acc);
+    acc = fmaf(1.0001f, tile_bilinear(T, (uint32_t)(int)u, __builtin_amdgcn_fractf(u), v), acc);
   }
   if (acc == 12345.678f) sink[0] = acc;
 }
 
-// Ray-coherent ceiling for the backprojector: bp_kernel's fast-path inner loop (per column one u
-// and weight per view, 16 z voxels whose rows advance by dv: clamp, bilinear sample from the LDS
-// window, weighted accumulate) over one resident 64 x 48 window per block, with the
-// backprojector's lane layout and view-to-view drift. No per-view projective set-up, no window
-// staging and no barriers, so it bounds the voxel-view rate of the LDS gather itself.
-// Bounds: u <= 2 + 10.5 + 4.5 + 0.35 < 62, v <= 2 + 3 + 1.5 + 0.35 + 15 * 1.3 < 46.
+// Ray-coherent ceiling for the backprojector: bp_kernel's fast-path inner loop for blocks inside
+// the volume (per column one u and weight per view, 16 z voxels whose rows advance by dv:
+// bilinear sample from the LDS window, weighted accumulate) over one resident 64 x 48 window per
+// block, with the backprojector's lane layout and view-to-view drift. No per-view projective
+// set-up, no window staging and no barriers, so it bounds the voxel-view rate of the LDS gather
+// itself. Bounds: u <= 2 + 10.5 + 4.5 + 0.35 < 62, v <= 2 + 3 + 1.5 + 0.35 + 15 * 1.3 < 46.
 __global__ __launch_bounds__(256) void gather_bench_bp_ray(const float* __restrict__ src,
                                                            int pitch, int nviews,
                                                            float* __restrict__ sink) {
   __shared__ float T[kTileCap];
   const float* g = src + (size_t)(blockIdx.x % 64) * 48 * pitch;
-  for (int e = threadIdx.x; e < kTileCap; e += 256) T[e] = g[(e / kTileW) * pitch + e % kTileW];
+  for (int e = threadIdx.x; e < kTileCap; e += 256)
+    T[e] = g[(e / kTileP) * pitch + min(e % kTileP, (uint32_t)kTileW - 1u)];
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const float xl = (float)((wave & 1) * 8 + (lane & 7)), yl = (float)((wave >> 1) * 8 + (lane >> 3));
@@ -357,8 +370,7 @@ __global__ __launch_bounds__(256) void gather_bench_bp_ray(const float* __restri
     const float au = __builtin_amdgcn_fractf(u);
 #pragma unroll
     for (int k = 0; k < kZR; ++k) {
-      const float v = clampf(fmaf((float)k, 1.3f, vb), 0.0f, 46.0f);
-      acc[k] = fmaf(wt, tile_bilinear(T, kTileW, iu, au, v), acc[k]);
+      acc[k] = fmaf(wt, tile_bilinear(T, iu, au, fmaf((float)k, 1.3f, vb)), acc[k]);
     }
   }
   float sum = 0.0f;
