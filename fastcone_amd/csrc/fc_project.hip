@@ -916,7 +916,12 @@ __global__ __launch_bounds__(kAT) void adjoint_vox_kernel(AdjVArgs a) {
     }
     if (t >= tend) break;
     const int p = g - start;
-    const int i = (int)(w.x >> 16) + p / Wj, j = (int)(w.x & 0xffffu) + p % Wj;
+    // p / Wj from a 1-ulp reciprocal, exact for p < 2^21: (p + 0.5) / Wj lies at least 0.5 / Wj
+    // from an integer and the roundings move it by less than 2^-2 / Wj (integer division for
+    // larger windows)
+    const int pi = p < (1 << 21) ? (int)(((float)p + 0.5f) * __builtin_amdgcn_rcpf((float)Wj))
+                                 : p / Wj;
+    const int i = (int)(w.x >> 16) + pi, j = (int)(w.x & 0xffffu) + (p - pi * Wj);
     const size_t ray = ((size_t)t * a.nh + i) * a.nw + j;
     const float4 r1 = a.rays[2 * ray + 1];
     if (r1.w == 0.0f) continue;  // zero datum or a ray that misses the block
@@ -930,7 +935,10 @@ __global__ __launch_bounds__(kAT) void adjoint_vox_kernel(AdjVArgs a) {
     float klo = 0.0f, khi = (float)(r.nk - 1);
     auto clip = [&](float p0, float st, float lo) {
       if (st != 0.0f) {
-        const float ka = (lo - p0) / st, kb = (lo + (float)kAB - p0) / st;
+        // a 1-ulp reciprocal: the range is widened by one step and every sample checked, and
+        // |ka| > 2^22 lies far outside [0, nk)
+        const float ist = __builtin_amdgcn_rcpf(st);
+        const float ka = (lo - p0) * ist, kb = (lo + (float)kAB - p0) * ist;
         klo = fmaxf(klo, fminf(ka, kb) - 1.0f);
         khi = fminf(khi, fmaxf(ka, kb) + 1.0f);
       } else if (!(p0 >= lo - 1.0f && p0 < lo + (float)kAB + 1.0f)) {
