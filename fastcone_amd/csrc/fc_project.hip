@@ -32,6 +32,7 @@ constexpr int kLdsCap = 6144;  // fp_lds_kernel's LDS brick budget, floats per b
 
 typedef float f32x2_a4 __attribute__((ext_vector_type(2), aligned(4)));
 typedef float f32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
+typedef uint32_t u32x3_a4 __attribute__((ext_vector_type(3), aligned(4)));
 
 enum FpLayout { FP_OCTET = 0, FP_PLAIN = 1, FP_LDS = 2 };
 
@@ -499,6 +500,34 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
           dst[1] = v.y;
           dst[2] = v.z;
           dst[3] = v.w;
+        }
+      }
+    } else if (DT != FC_F32 && X.gsd == 1u && dmax >= (uint32_t)(DP + 1)) {
+      // 16-bit storage, dominant axis x, two elements of slack after every row: the same
+      // four-element segments, each from three aligned dwords (the segment starts on either half
+      // of the first) shifted into place with alignbit, converted at the LDS store
+      constexpr int kSeg = (DP + 3) / 4, kRowsW = 64 / kSeg, kRowsB = NW * kRowsW;
+      const float invE1 = __builtin_amdgcn_rcpf((float)E1);  // exact as above
+      if (lane < kSeg * kRowsW) {
+        const uint32_t rr = (uint32_t)(((float)lane + 0.5f) * (1.0f / (float)kSeg));
+        const uint32_t sg = (uint32_t)lane - mul_u24(rr, (uint32_t)kSeg);
+        const uint32_t d0 = min(4u * sg, (uint32_t)(DP - 4));
+        const uint32_t* src = static_cast<const uint32_t*>(a.data);
+        const int nrow = E1 * E2;
+#pragma unroll 1
+        for (int rw = kRowsW * wave + (int)rr; rw < nrow; rw += kRowsB) {
+          const uint32_t m2 = (uint32_t)(((float)rw + 0.5f) * invE1);
+          const uint32_t m1 = (uint32_t)rw - mul_u24(m2, (uint32_t)E1);
+          const uint32_t e = gbase + d0 + mad_u24(m1, X.gs1, mul_u24(m2, X.gs2));
+          const u32x3_a4 q = *reinterpret_cast<const u32x3_a4*>(src + (e >> 1));
+          const uint32_t sh = (e & 1u) << 4;
+          const uint32_t e01 = __builtin_amdgcn_alignbit(q.y, q.x, sh);
+          const uint32_t e23 = __builtin_amdgcn_alignbit(q.z, q.y, sh);
+          float* dst = brick + mad_u24(m2, pad, mad_u24((uint32_t)rw, (uint32_t)DP, d0));
+          dst[0] = bits_to_f32<DT>(e01 & 0xffffu);
+          dst[1] = bits_to_f32<DT>(e01 >> 16);
+          dst[2] = bits_to_f32<DT>(e23 & 0xffffu);
+          dst[3] = bits_to_f32<DT>(e23 >> 16);
         }
       }
     } else if (X.gsd == 1u) {
