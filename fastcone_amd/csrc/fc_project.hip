@@ -13,8 +13,10 @@
 //          contiguously (32 B in FP32, 16 B in BF16/FP16), so one trilinear sample is one
 //          (FP32: two adjacent) 16-byte load. 8x the volume's bytes.
 //   PLAIN: the volume with a 2-voxel zero halo; one sample is four dword-aligned 8-byte loads
-//          of x-neighbour pairs. FP32 only.
-// The halo / padding makes every sample's corners in-bounds without per-tap checks.
+//          of x-neighbour pairs. FP32 only for global gathers.
+// The default projector (LDS-staged, below) copies bricks of the PLAIN staging, in any storage
+// type, into LDS and gathers from there. The halo / padding makes every sample's corners
+// in-bounds without per-tap checks.
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
@@ -27,7 +29,7 @@
 namespace fc {
 namespace {
 
-constexpr int kTile = 16;     // 16x16 pixel tile per 256-thread block, 8x8 per wave
+constexpr int kTile = 16;     // 16x16 pixel tiles, 8x8 pixels per wave
 constexpr int kLdsCap = 6144;  // fp_lds_kernel's LDS brick budget, floats per block (24 KiB)
 
 typedef float f32x2_a4 __attribute__((ext_vector_type(2), aligned(4)));
