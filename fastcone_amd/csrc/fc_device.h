@@ -115,6 +115,15 @@ __device__ __forceinline__ uint32_t mad_u24(uint32_t a, uint32_t b, uint32_t c) 
   asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
   return r;
 }
+// mad_u24 with a wave-uniform multiplier read from a scalar register (an operand the vector
+// form would copy into a VGPR at every use); b must be the same in every active lane.
+__device__ __forceinline__ uint32_t mad_u24_s(uint32_t a, uint32_t b, uint32_t c) {
+  uint32_t r;
+  asm("v_mad_u32_u24 %0, %1, %2, %3"
+      : "=v"(r)
+      : "v"(a), "s"(__builtin_amdgcn_readfirstlane(b)), "v"(c));
+  return r;
+}
 // a * b for a, b < 2^24 (kept as v_mul_u32_u24: the compiler would otherwise fold a product
 // with a constant into a quarter-rate 32-bit multiply)
 __device__ __forceinline__ uint32_t mul_u24(uint32_t a, uint32_t b) {

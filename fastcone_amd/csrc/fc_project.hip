@@ -239,14 +239,15 @@ __global__ __launch_bounds__(256) void fp_kernel(FpArgs a) {
 //     block's tile(s) between the slab planes (the frustum slice), widened by one voxel each side;
 //   * along the dominant axis it spans S + 3 planes, [lo - 1, lo + S + 1];
 //   * LDS layout: dominant axis fastest with compile-time pitch S + 3, so the 8 corners of a
-//     sample are two base addresses with immediate offsets {0, 1, S+3, S+4};
+//     sample are two base addresses with immediate offsets {0, 1, S+3, S+4} (plain values) or
+//     {0, S+3} (value / difference pairs);
 //   * a ray's samples are split between slabs by integer k ranges computed from the shared slab
 //     planes, so every sample is evaluated exactly once.
 // Work is dealt in "quads": one 16x16 pixel tile in four consecutive views. Adjacent views' rays
 // cross nearly the same voxels, so in "shared" mode each 512-thread block takes one 16x8 half of
 // the tile in all four views (one wave per view and 8x8 quarter) and its bricks are the union of
 // the four views' frustum slices over the half: one fill serves eight waves (fp_quad_kernel, two
-// blocks per quad, 8 waves per SIMD, bricks up to 32 KiB). A half shares its bricks between two
+// blocks per quad, 8 waves per SIMD, bricks up to 38 KiB). A half shares its bricks between two
 // quarters, which costs 30 % fewer brick elements per ray sample than a brick per quarter. When
 // the views are far apart (sparse angular sampling) the union is wide, and the quad is left to
 // fp_lds_kernel, where each 256-thread block takes the whole 16x16 tile of one view (6 waves per
@@ -259,7 +260,9 @@ __global__ __launch_bounds__(256) void fp_kernel(FpArgs a) {
 // axis) fall back to gathering from the staged volume in global memory, with the same arithmetic.
 
 constexpr int kQuad = 4;          // views per quad
-constexpr int kHalfCap = 8192;    // fp_quad_kernel's LDS brick budget, floats (32 KiB)
+// fp_quad_kernel's LDS brick budget, floats (38 KiB): with the slab table and the quad grid a
+// block takes 40528 bytes, so four 512-thread blocks (8 waves per SIMD) fill the CU's 160 KiB
+constexpr int kHalfCap = 9728;
 constexpr int kSlabTab = 64;      // slab boxes computed per table refill
 constexpr uint8_t kQuadShared = 1;
 
@@ -368,34 +371,75 @@ __device__ __forceinline__ int4 slab_box(const QuadGrid& G, const SlabAxes& X, i
   return make_int4(e1lo, e2lo, e1hi - e1lo + 1, e2hi - e2lo + 1);
 }
 
-// Plane pitch (floats) of an LDS brick of E1 rows of DP floats: DP E1 rounded up to 8 mod 16,
-// so consecutive planes start 8 or 24 of the 32 four-byte banks apart. A replay of the lanes'
-// addresses (8 x 8 rays per wave, 0.75-voxel spacing, the eight corner reads of each sample) on
-// 32 banks puts this among the lowest conflict counts of all plane offsets: 1.44 bank passes per
-// ideal pass against 1.97 unpadded.
+// LDS bricks hold (value, forward difference) pairs along the dominant axis whenever the pairs
+// fit the block's LDS budget: element (d, m1, m2) is the float2 (f(d), f(d + 1) - f(d)), so the
+// first interpolation step of a sample is one FMA per corner pair, f + a (f1 - f): the operations
+// of lerp(f, f1, a) in the same order (the subtraction done once, at the fill), so a sample's
+// value is bit-identical to interpolating plain values, for four fewer VALU instructions per
+// sample. A brick whose pairs would not fit holds plain values (one float per element) instead.
+// Rows hold DP elements (DP - 1 are sampled as pairs; the last pair's difference is never read),
+// planes E1 rows.
+//
+// Plane pitch (elements) of an LDS brick of E1 rows of DP elements: DP E1 rounded up to 8 mod 16.
+// Plain bricks: consecutive planes start 8 or 24 of the 32 four-byte banks apart; a replay of the
+// lanes' addresses (8 x 8 rays per wave, 0.75-voxel spacing, the eight corner reads of each
+// sample) on 32 banks puts this among the lowest conflict counts of all plane offsets: 1.44 bank
+// passes per ideal pass against 1.97 unpadded. Pair bricks: consecutive planes start 16 banks
+// apart; timed on the same lane layout (ds_read2_b64 of each plane's two corner pairs), the 16
+// plane offsets mod 16 pairs differ by up to 1.6x and 8 is the fastest.
 __host__ __device__ constexpr uint32_t brick_plane(uint32_t DP, uint32_t E1) {
   return DP * E1 + ((8u - DP * E1) & 15u);
 }
 
-// Trilinear sample from an LDS brick laid out dominant axis fastest (pitch DP), then minor axis 1
-// (rows), then minor axis 2 (planes of PSu floats = zbytes bytes). `base` is the brick address
-// minus the brick origin's offset, so the corner of a sample at brick-frame-relative coordinates
-// (qd, q1, q2) >= 0 is at index qd + DP q1 + PSu q2. Truncation is floor for q >= 0; the 24-bit
-// multiply-adds keep the index math full rate.
-template <int DP>
+// Elements (pairs or floats) of an E1 x E2 brick.
+__host__ __device__ constexpr uint32_t brick_elems(uint32_t DP, uint32_t E1, uint32_t E2) {
+  return brick_plane(DP, E1) * E2;
+}
+
+// Element e of a brick: the pair (f0, f1 - f0), or the plain value f0.
+template <bool PAIR>
+__device__ __forceinline__ void put_elem(float* brick, uint32_t e, float f0, float f1) {
+  if constexpr (PAIR) {
+    reinterpret_cast<float2*>(brick)[e] = make_float2(f0, f1 - f0);
+  } else {
+    brick[e] = f0;
+  }
+}
+
+// Trilinear sample from a brick laid out dominant axis fastest (pitch DP elements), then minor
+// axis 1 (rows), then minor axis 2 (planes of PSu elements = zbytes bytes). `base` is the brick
+// address minus the brick origin's offset, so the corner of a sample at brick-frame-relative
+// coordinates (qd, q1, q2) >= 0 is element qd + DP q1 + PSu q2 (PSu wave-uniform). Truncation is
+// floor for q >= 0; the 24-bit multiply-adds keep the index math full rate. Pair bricks: each
+// plane's two rows are one ds_read2_b64 (offsets 0 and DP pairs).
+template <int DP, bool PAIR>
 __device__ __forceinline__ float lds_trilinear(const char* base, uint32_t PSu, uint32_t zbytes,
                                                float qd, float q1, float q2) {
   const uint32_t id = (uint32_t)(int)qd, i1 = (uint32_t)(int)q1, i2 = (uint32_t)(int)q2;
   const float ad = __builtin_amdgcn_fractf(qd), a1f = __builtin_amdgcn_fractf(q1),
               a2f = __builtin_amdgcn_fractf(q2);
-  const uint32_t idx = mad_u24(i2, PSu, mad_u24(i1, (uint32_t)DP, id));
-  const float* L0 = reinterpret_cast<const float*>(base + 4u * idx);
-  const float* L1 = reinterpret_cast<const float*>(reinterpret_cast<const char*>(L0) + zbytes);
-  float c00 = lerp(L0[0], L0[1], ad), c10 = lerp(L0[DP], L0[DP + 1], ad);
-  float c01 = lerp(L1[0], L1[1], ad), c11 = lerp(L1[DP], L1[DP + 1], ad);
-  float c0 = lerp(c00, c10, a1f), c1 = lerp(c01, c11, a1f);
+  const uint32_t idx = mad_u24_s(i2, PSu, mad_u24(i1, (uint32_t)DP, id));
+  float c00, c10, c01, c11;
+  if constexpr (PAIR) {
+    const float2* L0 = reinterpret_cast<const float2*>(base + 8u * idx);
+    const float2* L1 = reinterpret_cast<const float2*>(reinterpret_cast<const char*>(L0) + zbytes);
+    const float2 e00 = L0[0], e10 = L0[DP], e01 = L1[0], e11 = L1[DP];
+    c00 = fmaf(ad, e00.y, e00.x); c10 = fmaf(ad, e10.y, e10.x);
+    c01 = fmaf(ad, e01.y, e01.x); c11 = fmaf(ad, e11.y, e11.x);
+  } else {
+    const float* L0 = reinterpret_cast<const float*>(base + 4u * idx);
+    const float* L1 = reinterpret_cast<const float*>(reinterpret_cast<const char*>(L0) + zbytes);
+    c00 = lerp(L0[0], L0[1], ad); c10 = lerp(L0[DP], L0[DP + 1], ad);
+    c01 = lerp(L1[0], L1[1], ad); c11 = lerp(L1[DP], L1[DP + 1], ad);
+  }
+  const float c0 = lerp(c00, c10, a1f), c1 = lerp(c01, c11, a1f);
   return lerp(c0, c1, a2f);
 }
+
+template <bool B>
+struct LayoutTag {
+  static constexpr bool pair = B;
+};
 
 // Sum of this thread's ray samples, slab by slab through LDS bricks whose boxes come from G
 // (views v0 .. v0 + nv - 1, sub-tile (cj0, ci0, cwj, cwi)), or from global memory when
@@ -455,7 +499,7 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
     if (E1 <= 0 || E2 <= 0) continue;  // the frustum misses the block here
     const int dlo = (int)lo - 1;
     const uint32_t PS = brick_plane(DP, (uint32_t)E1), pad = PS - (uint32_t)(DP * E1);
-    const int NE = (int)PS * E2;
+    const int NE = (int)brick_elems(DP, (uint32_t)E1, (uint32_t)E2);
     if (NE > lds_cap) {
       for (int k = kin; k < kout; ++k) {
         float kf = (float)k;
@@ -464,144 +508,167 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
       }
       continue;
     }
-    __syncthreads();  // the previous brick is no longer read
-    // Fill. Dominant axis x (contiguous in HBM and in the brick): each lane keeps one offset d
-    // along x and walks rows, so 11 consecutive lanes read one row's consecutive x and an
-    // element costs one row decomposition; four rows' loads are issued before their LDS
-    // writes. Dominant axis y or z: one thread per (minor1, minor2) column, minor axis 1
-    // being x so a wave's loads stay coalesced, and the DP elements along the dominant axis
-    // unrolled (LDS index d + DP c, pitch DP odd so the writes are bank-conflict free). Planes
-    // past the padded block (last slab) read its last plane and are never sampled.
-    // block-uniform: plain arithmetic keeps it on the scalar unit
-    const uint32_t gbase =
-        (uint32_t)dlo * X.gsd + (uint32_t)B.x * X.gs1 + (uint32_t)B.y * X.gs2;
-    const uint32_t dmax = (uint32_t)(X.PD - 1 - dlo);  // >= 2: dlo = b S <= PD - 3
-    if (DT == FC_F32 && X.gsd == 1u && dmax >= (uint32_t)(DP - 1)) {
-      // FP32, dominant axis x, every row whole: a row's DP elements are kSeg four-element
-      // segments starting at 4 s (the last at DP - 4, overlapping its neighbour: the shared
-      // elements are written twice with the same value), one 16-byte load per lane. Lanes
-      // 0 .. 3 kRowsW - 1 of each wave: segment lane % kSeg of row phase lane / kSeg.
-      constexpr int kSeg = (DP + 3) / 4, kRowsW = 64 / kSeg, kRowsB = NW * kRowsW;
-      // floor((r + 0.5) / E1) is exact for r < 2^13 with a 1-ulp reciprocal: (r + 0.5) / E1 lies
-      // at least 0.5 / E1 from an integer, and the two roundings move it by less than 2^-10 / E1
-      const float invE1 = __builtin_amdgcn_rcpf((float)E1);
-      if (lane < kSeg * kRowsW) {
-        const uint32_t rr = (uint32_t)(((float)lane + 0.5f) * (1.0f / (float)kSeg));
-        const uint32_t sg = (uint32_t)lane - mul_u24(rr, (uint32_t)kSeg);
-        const uint32_t d0 = min(4u * sg, (uint32_t)(DP - 4));
-        const float* src = static_cast<const float*>(a.data) + (gbase + d0);
-        const int nrow = E1 * E2;
+    // pairs when they fit the budget, plain values otherwise; block-uniform
+    auto slab = [&](auto tag) {
+      constexpr bool PAIR = decltype(tag)::pair;
+      __syncthreads();  // the previous brick is no longer read
+      // Fill: element (d, m1, m2) from the DP values d = 0 .. DP - 1 that each row loads along the
+      // dominant axis (a pair also takes value d + 1; the row's last pair, never sampled, gets
+      // difference 0 or the next value's). Dominant axis x (contiguous in HBM and in the brick):
+      // lanes keep one offset along x and walk rows, so consecutive lanes read one row's
+      // consecutive x and an element costs one row decomposition. Dominant axis y or z: one
+      // thread per (minor1, minor2) column, minor axis 1 being x so a wave's loads stay
+      // coalesced, and the DP elements along the dominant axis unrolled (LDS element d + DP c,
+      // pitch DP odd so the writes of consecutive columns spread over all banks). Planes past the
+      // padded block (last slab) read its last plane and are never sampled.
+      // block-uniform: plain arithmetic keeps it on the scalar unit
+      const uint32_t gbase =
+          (uint32_t)dlo * X.gsd + (uint32_t)B.x * X.gs1 + (uint32_t)B.y * X.gs2;
+      const uint32_t dmax = (uint32_t)(X.PD - 1 - dlo);  // >= 2: dlo = b S <= PD - 3
+      if (DT == FC_F32 && X.gsd == 1u && dmax >= (uint32_t)(DP - 1)) {
+        // FP32, dominant axis x, every row whole: a row's DP elements are kSeg four-element
+        // segments starting at 4 s (the last at DP - 4, overlapping its neighbour: the shared
+        // elements are written twice with the same value), one 16-byte load per lane, plus for
+        // pairs the value after the segment (the last segment's fourth pair is the row's last: it
+        // reloads its own value). Lanes 0 .. 3 kRowsW - 1 of each wave: segment lane % kSeg of
+        // row phase lane / kSeg.
+        constexpr int kSeg = (DP + 3) / 4, kRowsW = 64 / kSeg, kRowsB = NW * kRowsW;
+        // floor((r + 0.5) / E1) is exact for r < 2^13 with a 1-ulp reciprocal: (r + 0.5) / E1 lies
+        // at least 0.5 / E1 from an integer, and the two roundings move it by less than 2^-10 / E1
+        const float invE1 = __builtin_amdgcn_rcpf((float)E1);
+        if (lane < kSeg * kRowsW) {
+          const uint32_t rr = (uint32_t)(((float)lane + 0.5f) * (1.0f / (float)kSeg));
+          const uint32_t sg = (uint32_t)lane - mul_u24(rr, (uint32_t)kSeg);
+          const uint32_t d0 = min(4u * sg, (uint32_t)(DP - 4));
+          const uint32_t d4 = min(4u, (uint32_t)(DP - 1) - d0);
+          const float* src = static_cast<const float*>(a.data) + (gbase + d0);
+          const int nrow = E1 * E2;
 #pragma unroll 1
-        for (int rw = kRowsW * wave + (int)rr; rw < nrow; rw += kRowsB) {
-          const uint32_t m2 = (uint32_t)(((float)rw + 0.5f) * invE1);
-          const uint32_t m1 = (uint32_t)rw - mul_u24(m2, (uint32_t)E1);
-          const f32x4_a4 v =
-              *reinterpret_cast<const f32x4_a4*>(src + mad_u24(m1, X.gs1, mul_u24(m2, X.gs2)));
-          float* dst = brick + mad_u24(m2, pad, mad_u24((uint32_t)rw, (uint32_t)DP, d0));
-          dst[0] = v.x;
-          dst[1] = v.y;
-          dst[2] = v.z;
-          dst[3] = v.w;
+          for (int rw = kRowsW * wave + (int)rr; rw < nrow; rw += kRowsB) {
+            const uint32_t m2 = (uint32_t)(((float)rw + 0.5f) * invE1);
+            const uint32_t m1 = (uint32_t)rw - mul_u24(m2, (uint32_t)E1);
+            const uint32_t o = mad_u24(m1, X.gs1, mul_u24(m2, X.gs2));
+            const f32x4_a4 v = *reinterpret_cast<const f32x4_a4*>(src + o);
+            const float v4 = PAIR ? src[o + d4] : 0.0f;
+            const uint32_t e = mad_u24(m2, pad, mad_u24((uint32_t)rw, (uint32_t)DP, d0));
+            put_elem<PAIR>(brick, e, v.x, v.y);
+            put_elem<PAIR>(brick, e + 1u, v.y, v.z);
+            put_elem<PAIR>(brick, e + 2u, v.z, v.w);
+            put_elem<PAIR>(brick, e + 3u, v.w, v4);
+          }
         }
-      }
-    } else if (DT != FC_F32 && X.gsd == 1u && dmax >= (uint32_t)(DP + 1)) {
-      // 16-bit storage, dominant axis x, two elements of slack after every row: the same
-      // four-element segments, each from three aligned dwords (the segment starts on either half
-      // of the first) shifted into place with alignbit, converted at the LDS store
-      constexpr int kSeg = (DP + 3) / 4, kRowsW = 64 / kSeg, kRowsB = NW * kRowsW;
-      const float invE1 = __builtin_amdgcn_rcpf((float)E1);  // exact as above
-      if (lane < kSeg * kRowsW) {
-        const uint32_t rr = (uint32_t)(((float)lane + 0.5f) * (1.0f / (float)kSeg));
-        const uint32_t sg = (uint32_t)lane - mul_u24(rr, (uint32_t)kSeg);
-        const uint32_t d0 = min(4u * sg, (uint32_t)(DP - 4));
-        const uint32_t* src = static_cast<const uint32_t*>(a.data);
-        const int nrow = E1 * E2;
+      } else if (DT != FC_F32 && X.gsd == 1u && dmax >= (uint32_t)(DP + 1)) {
+        // 16-bit storage, dominant axis x, two elements of slack after every row: the same
+        // four-element segments, each from three aligned dwords (the segment starts on either half
+        // of the first) shifted into place with alignbit (pairs: and the fifth value from the
+        // third dword), converted at the LDS store
+        constexpr int kSeg = (DP + 3) / 4, kRowsW = 64 / kSeg, kRowsB = NW * kRowsW;
+        const float invE1 = __builtin_amdgcn_rcpf((float)E1);  // exact as above
+        if (lane < kSeg * kRowsW) {
+          const uint32_t rr = (uint32_t)(((float)lane + 0.5f) * (1.0f / (float)kSeg));
+          const uint32_t sg = (uint32_t)lane - mul_u24(rr, (uint32_t)kSeg);
+          const uint32_t d0 = min(4u * sg, (uint32_t)(DP - 4));
+          const uint32_t* src = static_cast<const uint32_t*>(a.data);
+          const int nrow = E1 * E2;
 #pragma unroll 1
-        for (int rw = kRowsW * wave + (int)rr; rw < nrow; rw += kRowsB) {
-          const uint32_t m2 = (uint32_t)(((float)rw + 0.5f) * invE1);
-          const uint32_t m1 = (uint32_t)rw - mul_u24(m2, (uint32_t)E1);
-          const uint32_t e = gbase + d0 + mad_u24(m1, X.gs1, mul_u24(m2, X.gs2));
-          const u32x3_a4 q = *reinterpret_cast<const u32x3_a4*>(src + (e >> 1));
-          const uint32_t sh = (e & 1u) << 4;
-          const uint32_t e01 = __builtin_amdgcn_alignbit(q.y, q.x, sh);
-          const uint32_t e23 = __builtin_amdgcn_alignbit(q.z, q.y, sh);
-          float* dst = brick + mad_u24(m2, pad, mad_u24((uint32_t)rw, (uint32_t)DP, d0));
-          dst[0] = bits_to_f32<DT>(e01 & 0xffffu);
-          dst[1] = bits_to_f32<DT>(e01 >> 16);
-          dst[2] = bits_to_f32<DT>(e23 & 0xffffu);
-          dst[3] = bits_to_f32<DT>(e23 >> 16);
+          for (int rw = kRowsW * wave + (int)rr; rw < nrow; rw += kRowsB) {
+            const uint32_t m2 = (uint32_t)(((float)rw + 0.5f) * invE1);
+            const uint32_t m1 = (uint32_t)rw - mul_u24(m2, (uint32_t)E1);
+            const uint32_t e = gbase + d0 + mad_u24(m1, X.gs1, mul_u24(m2, X.gs2));
+            const u32x3_a4 q = *reinterpret_cast<const u32x3_a4*>(src + (e >> 1));
+            const uint32_t sh = (e & 1u) << 4;
+            const uint32_t e01 = __builtin_amdgcn_alignbit(q.y, q.x, sh);
+            const uint32_t e23 = __builtin_amdgcn_alignbit(q.z, q.y, sh);
+            const float f0 = bits_to_f32<DT>(e01 & 0xffffu), f1 = bits_to_f32<DT>(e01 >> 16);
+            const float f2 = bits_to_f32<DT>(e23 & 0xffffu), f3 = bits_to_f32<DT>(e23 >> 16);
+            const float f4 = bits_to_f32<DT>((q.z >> sh) & 0xffffu);
+            const uint32_t w = mad_u24(m2, pad, mad_u24((uint32_t)rw, (uint32_t)DP, d0));
+            put_elem<PAIR>(brick, w, f0, f1);
+            put_elem<PAIR>(brick, w + 1u, f1, f2);
+            put_elem<PAIR>(brick, w + 2u, f2, f3);
+            put_elem<PAIR>(brick, w + 3u, f3, f4);
+          }
         }
-      }
-    } else if (X.gsd == 1u) {
-      // lanes 0..54 of each wave: d = lane % DP (dominant offset) and row phase lane / DP; the
-      // block's NW waves cover 5 NW consecutive rows per step
-      const float invE1 = __builtin_amdgcn_rcpf((float)E1);  // exact as above
-      constexpr int kLanes = (64 / DP) * DP, kRowsW = 64 / DP, kRowsB = NW * kRowsW;
-      if (lane < kLanes) {
-        const uint32_t rr = (uint32_t)(((float)lane + 0.5f) * (1.0f / (float)DP));
-        const uint32_t d = (uint32_t)lane - mul_u24(rr, (uint32_t)DP);
-        const uint32_t gd = gbase + min(d, dmax);
-        const int nrow = E1 * E2;
+      } else if (X.gsd == 1u) {
+        // lanes 0..54 of each wave: d = lane % DP (dominant offset) and row phase lane / DP; the
+        // block's NW waves cover 5 NW consecutive rows per step; for pairs each lane also loads
+        // the next value (clamped to the padded row)
+        const float invE1 = __builtin_amdgcn_rcpf((float)E1);  // exact as above
+        constexpr int kLanes = (64 / DP) * DP, kRowsW = 64 / DP, kRowsB = NW * kRowsW;
+        if (lane < kLanes) {
+          const uint32_t rr = (uint32_t)(((float)lane + 0.5f) * (1.0f / (float)DP));
+          const uint32_t d = (uint32_t)lane - mul_u24(rr, (uint32_t)DP);
+          const uint32_t gd = gbase + min(d, dmax), gd1 = gbase + min(d + 1u, dmax);
+          const int nrow = E1 * E2;
 #pragma unroll 1
-        for (int r0 = kRowsW * wave + (int)rr; r0 < nrow; r0 += 4 * kRowsB) {
-          uint32_t v[4], w[4];
+          for (int r0 = kRowsW * wave + (int)rr; r0 < nrow; r0 += 4 * kRowsB) {
+            uint32_t v[4], v1[4], w[4];
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int rw = r0 + q * kRowsB;
-            v[q] = 0u;
-            w[q] = 0u;
-            if (rw < nrow) {
-              const uint32_t m2 = (uint32_t)(((float)rw + 0.5f) * invE1);
-              const uint32_t m1 = (uint32_t)rw - mul_u24(m2, (uint32_t)E1);
-              v[q] = load_bits<DT>(a.data, mad_u24(m1, X.gs1, mad_u24(m2, X.gs2, gd)));
-              w[q] = mad_u24(m2, pad, mad_u24((uint32_t)rw, (uint32_t)DP, d));
+            for (int q = 0; q < 4; ++q) {
+              const int rw = r0 + q * kRowsB;
+              v[q] = 0u;
+              v1[q] = 0u;
+              w[q] = 0u;
+              if (rw < nrow) {
+                const uint32_t m2 = (uint32_t)(((float)rw + 0.5f) * invE1);
+                const uint32_t m1 = (uint32_t)rw - mul_u24(m2, (uint32_t)E1);
+                const uint32_t o = mad_u24(m1, X.gs1, mul_u24(m2, X.gs2));
+                v[q] = load_bits<DT>(a.data, gd + o);
+                if constexpr (PAIR) v1[q] = load_bits<DT>(a.data, gd1 + o);
+                w[q] = mad_u24(m2, pad, mad_u24((uint32_t)rw, (uint32_t)DP, d));
+              }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int rw = r0 + q * kRowsB;
+              if (rw < nrow)
+                put_elem<PAIR>(brick, w[q], bits_to_f32<DT>(v[q]), bits_to_f32<DT>(v1[q]));
             }
           }
+        }
+      } else {
+        const int ncol = E1 * E2;
+        const float invE1 = __builtin_amdgcn_rcpf((float)E1);  // exact as above
+#pragma unroll 1
+        for (int c = threadIdx.x; c < ncol; c += NT) {
+          const uint32_t m2 = (uint32_t)(((float)c + 0.5f) * invE1);  // exact: c < 2^10
+          const uint32_t m1 = (uint32_t)c - mul_u24(m2, (uint32_t)E1);
+          const uint32_t g = mad_u24(m1, X.gs1, mad_u24(m2, X.gs2, gbase));
+          float v[DP];
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int rw = r0 + q * kRowsB;
-            if (rw < nrow) brick[w[q]] = bits_to_f32<DT>(v[q]);
-          }
+          for (int d = 0; d < DP; ++d)
+            v[d] = load_elem<DT>(a.data, mad_u24(min((uint32_t)d, dmax), X.gsd, g));
+          const uint32_t col = mad_u24(m2, pad, mul_u24((uint32_t)c, (uint32_t)DP));
+#pragma unroll
+          for (int d = 0; d < DP; ++d)
+            put_elem<PAIR>(brick, col + d, v[d], v[d < DP - 1 ? d + 1 : d]);
         }
       }
-    } else {
-      const int ncol = E1 * E2;
-      const float invE1 = __builtin_amdgcn_rcpf((float)E1);  // exact as above
-#pragma unroll 1
-      for (int c = threadIdx.x; c < ncol; c += NT) {
-        const uint32_t m2 = (uint32_t)(((float)c + 0.5f) * invE1);  // exact: c < 2^10
-        const uint32_t m1 = (uint32_t)c - mul_u24(m2, (uint32_t)E1);
-        const uint32_t g = mad_u24(m1, X.gs1, mad_u24(m2, X.gs2, gbase));
-        float v[DP];
-#pragma unroll
-        for (int d = 0; d < DP; ++d)
-          v[d] = load_elem<DT>(a.data, mad_u24(min((uint32_t)d, dmax), X.gsd, g));
-        float* col = brick + mad_u24(m2, pad, mul_u24((uint32_t)c, (uint32_t)DP));
-#pragma unroll
-        for (int d = 0; d < DP; ++d) col[d] = v[d];
+      __syncthreads();
+      // Brick-relative LDS byte offset of a sample's corner (0,0,0). Sample coordinates are >= 0
+      // in the padded frame (ray_segment guarantees it), so truncation is floor; the 24-bit
+      // multiply-adds stay full rate.
+      const uint32_t boff = (uint32_t)(dlo + DP * B.x) + PS * (uint32_t)B.y;
+      constexpr uint32_t EB = PAIR ? 8u : 4u;  // bytes per element
+      const uint32_t zbytes = EB * PS;  // one m2 plane
+      const char* base = reinterpret_cast<const char*>(brick) - EB * boff;
+      auto sample = [&](float kf) {
+        return lds_trilinear<DP, PAIR>(base, PS, zbytes, fmaf(kf, sd, pd0), fmaf(kf, s1, p10),
+                                       fmaf(kf, s2, p20));
+      };
+      // two samples per iteration: both samples' LDS reads are in flight together; the loop
+      // counts in FP32 (exact: k < 2^24), so one add and one compare serve both samples
+      const float kend = (float)kout;
+      float kf = (float)kin, kf1 = kf + 1.0f;
+      for (; kf1 < kend; kf += 2.0f, kf1 += 2.0f) {
+        const float q0 = sample(kf), q1 = sample(kf1);
+        acc += q0;
+        acc += q1;
       }
-    }
-    __syncthreads();
-    // Brick-relative LDS byte offset of a sample's corner (0,0,0). Sample coordinates are >= 0
-    // in the padded frame (ray_segment guarantees it), so truncation is floor; the 24-bit
-    // multiply-adds stay full rate.
-    const uint32_t boff = (uint32_t)(dlo + DP * B.x) + PS * (uint32_t)B.y;
-    const uint32_t zbytes = 4u * PS;  // one m2 plane
-    const char* base = reinterpret_cast<const char*>(brick) - 4u * boff;
-    auto sample = [&](float kf) {
-      return lds_trilinear<DP>(base, PS, zbytes, fmaf(kf, sd, pd0), fmaf(kf, s1, p10),
-                               fmaf(kf, s2, p20));
+      if (kf < kend) acc += sample(kf);
     };
-    // two samples per iteration: both samples' LDS reads are in flight together; the loop
-    // counts in FP32 (exact: k < 2^24), so one add and one compare serve both samples
-    const float kend = (float)kout;
-    float kf = (float)kin, kf1 = kf + 1.0f;
-    for (; kf1 < kend; kf += 2.0f, kf1 += 2.0f) {
-      const float q0 = sample(kf), q1 = sample(kf1);
-      acc += q0;
-      acc += q1;
-    }
-    if (kf < kend) acc += sample(kf);
+    if (2 * NE <= lds_cap) slab(LayoutTag<true>{});
+    else slab(LayoutTag<false>{});
   }
   return acc;
 }
@@ -651,8 +718,7 @@ void fp_quad_kernel(FpArgs a, uint8_t* __restrict__ quad_mode, int mode) {
       const int4 B = kind < 2 ? slab_box<S>(G, X, 0, kQuad, 0, kind * 2, 2, 0, m)
                               : slab_box<S>(G, X, kind - 2, 1, 0, 0, 2, 2, m);
       if (B.z > 0 && B.w > 0) {
-        const unsigned long long ne =
-            (unsigned long long)brick_plane(DP, (uint32_t)B.z) * (unsigned)B.w;
+        const unsigned long long ne = brick_elems(DP, (uint32_t)B.z, (uint32_t)B.w);
         const int cap = kind < 2 ? cap_shared : a.lds_cap;
         const unsigned long long c = ne > (unsigned long long)cap ? 4ull * ne : ne;
         if (kind < 2) csh += c; else csi += c;
@@ -1155,14 +1221,14 @@ __global__ __launch_bounds__(256) void gather_bench_fp(FpArgs a, int B, int nsam
   if (acc == 12345.678f) sink[0] = acc;  // keep the work alive
 }
 
-// LDS variant: each block stages one DP x E x E brick of the plain staged volume in LDS (same
-// layout as fp_lds_kernel), then draws trilinear samples at pseudo-random positions inside it
+// LDS variant: each block stages one DP x E x E pair brick of the plain staged volume in LDS
+// (same layout as fp_lds_kernel), then draws trilinear samples at pseudo-random positions inside it
 // with the projector's own LDS fetch and interpolation (two samples per iteration).
 template <int S>
 __global__ __launch_bounds__(256) void gather_bench_lds(FpArgs a, int E, int nsamp,
                                                         float* __restrict__ sink) {
   constexpr int DP = S + 3;
-  __shared__ float brick[kLdsCap];
+  __shared__ float brick[2 * kLdsCap];  // pair bricks up to E = 23
   const uint32_t PS = brick_plane(DP, (uint32_t)E);
   const int NE = DP * E * E;
   const int nb = (a.nx - E) / E;
@@ -1171,14 +1237,15 @@ __global__ __launch_bounds__(256) void gather_bench_lds(FpArgs a, int E, int nsa
                       (uint32_t)(2 + (b / nb) * E) * (uint32_t)a.PX * (uint32_t)a.PY + 2u;
   for (int e = threadIdx.x; e < NE; e += 256) {
     const int d = e % DP, rest = e / DP, m1 = rest % E, m2 = rest / E;
-    brick[d + DP * m1 + PS * m2] = static_cast<const float*>(a.data)[g0 + d + (uint32_t)m1 * a.PX +
-                                                                     (uint32_t)m2 * a.PX * a.PY];
+    const float* src = static_cast<const float*>(a.data) + g0 + d + (uint32_t)m1 * a.PX +
+                       (uint32_t)m2 * a.PX * a.PY;
+    put_elem<true>(brick, d + DP * m1 + PS * m2, src[0], src[1]);
   }
   __syncthreads();
   uint32_t s = 0x9e3779b9u * (blockIdx.x * 256 + threadIdx.x + 1);
   const float sd = (float)(DP - 1) * (1.0f / 16777216.0f), se = (float)(E - 1) * (1.0f / 16777216.0f);
   const char* base = reinterpret_cast<const char*>(brick);
-  const uint32_t zbytes = 4u * PS;
+  const uint32_t zbytes = 8u * PS;
   float acc = 0.0f;
   for (int k = 0; k < nsamp; k += 2) {
     float q[6];
@@ -1187,8 +1254,8 @@ __global__ __launch_bounds__(256) void gather_bench_lds(FpArgs a, int E, int nsa
       s = s * 1664525u + 1013904223u;
       q[c] = (float)(s >> 8) * ((c % 3) == 0 ? sd : se);
     }
-    const float r0 = lds_trilinear<DP>(base, PS, zbytes, q[0], q[1], q[2]);
-    const float r1 = lds_trilinear<DP>(base, PS, zbytes, q[3], q[4], q[5]);
+    const float r0 = lds_trilinear<DP, true>(base, PS, zbytes, q[0], q[1], q[2]);
+    const float r1 = lds_trilinear<DP, true>(base, PS, zbytes, q[3], q[4], q[5]);
     acc += r0;
     acc += r1;
   }
@@ -1196,11 +1263,11 @@ __global__ __launch_bounds__(256) void gather_bench_lds(FpArgs a, int E, int nsa
 }
 
 // Ray-coherent LDS gather ceiling: the projector's trilinear LDS sample (same lds_trilinear,
-// positions p0 + k s by FMA) over one resident 11 x 16 x 16 brick (padded planes, as in the
+// positions p0 + k s by FMA) over one resident 11 x 16 x 16 pair brick (padded planes, as in the
 // projector), with the projector's lane layout (8 x 8 rays per wave, 16 x 16 per block) and
 // ray-like positions (0.75-voxel pixel spacing, diverging directions, 16 samples per slab pass),
 // U samples per iteration. The LDS
-// allocation is exactly the brick (11.5 KiB), so occupancy is limited by registers only. No refill,
+// allocation is exactly the pair brick (23 KiB). No refill,
 // no ray set-up and no divergence: it bounds the tap rate of LDS trilinear sampling along rays.
 // Bounds: qd <= 1 + 15 * 0.47 < 9, q1, q2 in [1.25 - 1.125, 1.25 + 11.25 + 1.125 + 0.25] =
 // [0.125, 13.875], so every corner index stays in [0, 9] x [0, 14] x [0, 14].
@@ -1209,15 +1276,16 @@ __global__ __launch_bounds__(256) void gather_bench_lds_ray(FpArgs a, int reps,
                                                             float* __restrict__ sink) {
   constexpr int DP = S + 3, E = 16, NE = DP * E * E;
   constexpr uint32_t PS = brick_plane(DP, E);
-  __shared__ float brick[PS * E];
+  __shared__ float brick[2 * brick_elems(DP, E, E)];
   const int nb = (a.nx - E) / E;
   const int b = blockIdx.x % (nb * nb);
   const uint32_t g0 = (uint32_t)(2 + (b % nb) * E) * (uint32_t)a.PX +
                       (uint32_t)(2 + (b / nb) * E) * (uint32_t)a.PX * (uint32_t)a.PY + 2u;
   for (int e = threadIdx.x; e < NE; e += 256) {
     const int d = e % DP, rest = e / DP, m1 = rest % E, m2 = rest / E;
-    brick[d + DP * m1 + PS * m2] = static_cast<const float*>(a.data)[g0 + d + (uint32_t)m1 * a.PX +
-                                                                     (uint32_t)m2 * a.PX * a.PY];
+    const float* src = static_cast<const float*>(a.data) + g0 + d + (uint32_t)m1 * a.PX +
+                       (uint32_t)m2 * a.PX * a.PY;
+    put_elem<true>(brick, d + DP * m1 + PS * m2, src[0], src[1]);
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1225,7 +1293,7 @@ __global__ __launch_bounds__(256) void gather_bench_lds_ray(FpArgs a, int reps,
   const float sd = 0.47f, s1 = 0.01f * (jj - 7.5f), s2 = 0.01f * (ii - 7.5f);
   const float pd0 = 1.0f;
   const char* base = reinterpret_cast<const char*>(brick);
-  const uint32_t zbytes = 4u * PS;
+  const uint32_t zbytes = 8u * PS;
   float acc = 0.0f;
 #pragma unroll 1
   for (int rep = 0; rep < reps; ++rep) {
@@ -1237,7 +1305,7 @@ __global__ __launch_bounds__(256) void gather_bench_lds_ray(FpArgs a, int reps,
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const float kf = (float)(k + u);
-        r[u] = lds_trilinear<DP>(base, PS, zbytes, fmaf(kf, sd, pd0),
+        r[u] = lds_trilinear<DP, true>(base, PS, zbytes, fmaf(kf, sd, pd0),
                                  fmaf(kf, s1, p10), fmaf(kf, s2, p20));
       }
 #pragma unroll
