@@ -436,6 +436,29 @@ __device__ __forceinline__ float lds_trilinear(const char* base, uint32_t PSu, u
   return lerp(c0, c1, a2f);
 }
 
+// The four corner pairs of a sample's cell in a pair brick: rows i1 and i1 + 1 of planes i2 and
+// i2 + 1 (element idx = id + DP i1 + PSu i2 and its neighbours), and the trilinear value at the
+// cell fractions (ad, a1f, a2f) — the arithmetic of lds_trilinear<DP, true>.
+struct CellPairs {
+  float2 e00, e10, e01, e11;
+};
+
+template <int DP>
+__device__ __forceinline__ void load_cell(const char* base, uint32_t idx, uint32_t zbytes,
+                                          CellPairs& c) {
+  const float2* L0 = reinterpret_cast<const float2*>(base + 8u * idx);
+  const float2* L1 = reinterpret_cast<const float2*>(reinterpret_cast<const char*>(L0) + zbytes);
+  c.e00 = L0[0]; c.e10 = L0[DP]; c.e01 = L1[0]; c.e11 = L1[DP];
+}
+
+__device__ __forceinline__ float cell_trilinear(const CellPairs& c, float ad, float a1f,
+                                                float a2f) {
+  const float c00 = fmaf(ad, c.e00.y, c.e00.x), c10 = fmaf(ad, c.e10.y, c.e10.x);
+  const float c01 = fmaf(ad, c.e01.y, c.e01.x), c11 = fmaf(ad, c.e11.y, c.e11.x);
+  const float c0 = lerp(c00, c10, a1f), c1 = lerp(c01, c11, a1f);
+  return lerp(c0, c1, a2f);
+}
+
 template <bool B>
 struct LayoutTag {
   static constexpr bool pair = B;
@@ -652,20 +675,42 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
       constexpr uint32_t EB = PAIR ? 8u : 4u;  // bytes per element
       const uint32_t zbytes = EB * PS;  // one m2 plane
       const char* base = reinterpret_cast<const char*>(brick) - EB * boff;
-      auto sample = [&](float kf) {
-        return lds_trilinear<DP, PAIR>(base, PS, zbytes, fmaf(kf, sd, pd0), fmaf(kf, s1, p10),
-                                       fmaf(kf, s2, p20));
-      };
-      // two samples per iteration: both samples' LDS reads are in flight together; the loop
-      // counts in FP32 (exact: k < 2^24), so one add and one compare serve both samples
+      // two samples per iteration; the loop counts in FP32 (exact: k < 2^24), so one add and one
+      // compare serve both samples
       const float kend = (float)kout;
       float kf = (float)kin, kf1 = kf + 1.0f;
-      for (; kf1 < kend; kf += 2.0f, kf1 += 2.0f) {
-        const float q0 = sample(kf), q1 = sample(kf1);
-        acc += q0;
-        acc += q1;
+      if constexpr (PAIR) {
+        // A lane reads its cell's corner pairs only when the sample's base cell differs from its
+        // previous sample's (a masked LDS read): samples 0.5 voxel apart along a ray share a cell
+        // about half the time, and the LDS is the bound. The first sample of a slab always reads
+        // (no element index is 2^32 - 1).
+        CellPairs C;
+        C.e00 = C.e10 = C.e01 = C.e11 = make_float2(0.0f, 0.0f);
+        uint32_t cur = 0xffffffffu;
+        auto step = [&](float k) {
+          const float qd = fmaf(k, sd, pd0), q1 = fmaf(k, s1, p10), q2 = fmaf(k, s2, p20);
+          const uint32_t idx =
+              mad_u24_s((uint32_t)(int)q2, PS, mad_u24((uint32_t)(int)q1, (uint32_t)DP,
+                                                       (uint32_t)(int)qd));
+          if (idx != cur) load_cell<DP>(base, idx, zbytes, C);
+          cur = idx;
+          acc += cell_trilinear(C, __builtin_amdgcn_fractf(qd), __builtin_amdgcn_fractf(q1),
+                                __builtin_amdgcn_fractf(q2));
+        };
+        for (; kf < kend; kf += 1.0f) step(kf);
+      } else {
+        auto sample = [&](float k) {
+          return lds_trilinear<DP, false>(base, PS, zbytes, fmaf(k, sd, pd0), fmaf(k, s1, p10),
+                                          fmaf(k, s2, p20));
+        };
+        // both samples' LDS reads are in flight together
+        for (; kf1 < kend; kf += 2.0f, kf1 += 2.0f) {
+          const float q0 = sample(kf), q1 = sample(kf1);
+          acc += q0;
+          acc += q1;
+        }
+        if (kf < kend) acc += sample(kf);
       }
-      if (kf < kend) acc += sample(kf);
     };
     if (2 * NE <= lds_cap) slab(LayoutTag<true>{});
     else slab(LayoutTag<false>{});
