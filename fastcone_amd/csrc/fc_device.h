@@ -124,6 +124,12 @@ __device__ __forceinline__ uint32_t mad_u24_s(uint32_t a, uint32_t b, uint32_t c
       : "v"(a), "s"(__builtin_amdgcn_readfirstlane(b)), "v"(c));
   return r;
 }
+// a[23:0] * b[23:0] (low 32 bits) with a wave-uniform b read from a scalar register.
+__device__ __forceinline__ uint32_t mul_u24_s(uint32_t a, uint32_t b) {
+  uint32_t r;
+  asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "s"(__builtin_amdgcn_readfirstlane(b)), "v"(a));
+  return r;
+}
 // a * b for a, b < 2^24 (kept as v_mul_u32_u24: the compiler would otherwise fold a product
 // with a constant into a quarter-rate 32-bit multiply)
 __device__ __forceinline__ uint32_t mul_u24(uint32_t a, uint32_t b) {

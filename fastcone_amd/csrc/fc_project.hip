@@ -569,11 +569,11 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
 #pragma unroll 1
           for (int rw = kRowsW * wave + (int)rr; rw < nrow; rw += kRowsB) {
             const uint32_t m2 = (uint32_t)(((float)rw + 0.5f) * invE1);
-            const uint32_t m1 = (uint32_t)rw - mul_u24(m2, (uint32_t)E1);
-            const uint32_t o = mad_u24(m1, X.gs1, mul_u24(m2, X.gs2));
+            const uint32_t m1 = (uint32_t)rw - mul_u24_s(m2, (uint32_t)E1);
+            const uint32_t o = mad_u24_s(m1, X.gs1, mul_u24_s(m2, X.gs2));
             const f32x4_a4 v = *reinterpret_cast<const f32x4_a4*>(src + o);
             const float v4 = PAIR ? src[o + d4] : 0.0f;
-            const uint32_t e = mad_u24(m2, pad, mad_u24((uint32_t)rw, (uint32_t)DP, d0));
+            const uint32_t e = mad_u24_s(m2, pad, mad_u24((uint32_t)rw, (uint32_t)DP, d0));
             put_elem<PAIR>(brick, e, v.x, v.y);
             put_elem<PAIR>(brick, e + 1u, v.y, v.z);
             put_elem<PAIR>(brick, e + 2u, v.z, v.w);
@@ -596,8 +596,8 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
 #pragma unroll 1
           for (int rw = kRowsW * wave + (int)rr; rw < nrow; rw += kRowsB) {
             const uint32_t m2 = (uint32_t)(((float)rw + 0.5f) * invE1);
-            const uint32_t m1 = (uint32_t)rw - mul_u24(m2, (uint32_t)E1);
-            const uint32_t e = gbase + d0 + mad_u24(m1, X.gs1, mul_u24(m2, X.gs2));
+            const uint32_t m1 = (uint32_t)rw - mul_u24_s(m2, (uint32_t)E1);
+            const uint32_t e = gbase + d0 + mad_u24_s(m1, X.gs1, mul_u24_s(m2, X.gs2));
             const u32x3_a4 q = *reinterpret_cast<const u32x3_a4*>(src + (e >> 1));
             const uint32_t sh = (e & 1u) << 4;
             const uint32_t e01 = __builtin_amdgcn_alignbit(q.y, q.x, sh);
@@ -605,7 +605,7 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
             const float f0 = bits_to_f32<DT>(e01 & 0xffffu), f1 = bits_to_f32<DT>(e01 >> 16);
             const float f2 = bits_to_f32<DT>(e23 & 0xffffu), f3 = bits_to_f32<DT>(e23 >> 16);
             const float f4 = bits_to_f32<DT>((q.z >> sh) & 0xffffu);
-            const uint32_t w = mad_u24(m2, pad, mad_u24((uint32_t)rw, (uint32_t)DP, d0));
+            const uint32_t w = mad_u24_s(m2, pad, mad_u24((uint32_t)rw, (uint32_t)DP, d0));
             put_elem<PAIR>(brick, w, f0, f1);
             put_elem<PAIR>(brick, w + 1u, f1, f2);
             put_elem<PAIR>(brick, w + 2u, f2, f3);
@@ -634,11 +634,11 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
               w[q] = 0u;
               if (rw < nrow) {
                 const uint32_t m2 = (uint32_t)(((float)rw + 0.5f) * invE1);
-                const uint32_t m1 = (uint32_t)rw - mul_u24(m2, (uint32_t)E1);
-                const uint32_t o = mad_u24(m1, X.gs1, mul_u24(m2, X.gs2));
+                const uint32_t m1 = (uint32_t)rw - mul_u24_s(m2, (uint32_t)E1);
+                const uint32_t o = mad_u24_s(m1, X.gs1, mul_u24_s(m2, X.gs2));
                 v[q] = load_bits<DT>(a.data, gd + o);
                 if constexpr (PAIR) v1[q] = load_bits<DT>(a.data, gd1 + o);
-                w[q] = mad_u24(m2, pad, mad_u24((uint32_t)rw, (uint32_t)DP, d));
+                w[q] = mad_u24_s(m2, pad, mad_u24((uint32_t)rw, (uint32_t)DP, d));
               }
             }
 #pragma unroll
@@ -655,13 +655,17 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
 #pragma unroll 1
         for (int c = threadIdx.x; c < ncol; c += NT) {
           const uint32_t m2 = (uint32_t)(((float)c + 0.5f) * invE1);  // exact: c < 2^10
-          const uint32_t m1 = (uint32_t)c - mul_u24(m2, (uint32_t)E1);
-          const uint32_t g = mad_u24(m1, X.gs1, mad_u24(m2, X.gs2, gbase));
+          const uint32_t m1 = (uint32_t)c - mul_u24_s(m2, (uint32_t)E1);
+          const uint32_t g = gbase + mad_u24_s(m1, X.gs1, mul_u24_s(m2, X.gs2));
+          // the column's address once; each element adds a wave-uniform offset
+          const void* colp =
+              static_cast<const char*>(a.data) + (size_t)g * (DT == FC_F32 ? 4u : 2u);
           float v[DP];
 #pragma unroll
           for (int d = 0; d < DP; ++d)
-            v[d] = load_elem<DT>(a.data, mad_u24(min((uint32_t)d, dmax), X.gsd, g));
-          const uint32_t col = mad_u24(m2, pad, mul_u24((uint32_t)c, (uint32_t)DP));
+            v[d] = load_elem<DT>(colp, (size_t)(uint32_t)__builtin_amdgcn_readfirstlane(
+                                           min((uint32_t)d, dmax) * X.gsd));
+          const uint32_t col = mad_u24_s(m2, pad, mul_u24((uint32_t)c, (uint32_t)DP));
 #pragma unroll
           for (int d = 0; d < DP; ++d)
             put_elem<PAIR>(brick, col + d, v[d], v[d < DP - 1 ? d + 1 : d]);
@@ -694,10 +698,21 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
                                                        (uint32_t)(int)qd));
           if (idx != cur) load_cell<DP>(base, idx, zbytes, C);
           cur = idx;
-          acc += cell_trilinear(C, __builtin_amdgcn_fractf(qd), __builtin_amdgcn_fractf(q1),
+          return cell_trilinear(C, __builtin_amdgcn_fractf(qd), __builtin_amdgcn_fractf(q1),
                                 __builtin_amdgcn_fractf(q2));
         };
-        for (; kf < kend; kf += 1.0f) step(kf);
+        // Two steps per iteration. The empty asm (a use of the first value, ordered against
+        // memory) keeps the first step's interpolation ahead of the second step's reads, so both
+        // steps keep the pairs in one set of registers (moved down, the first step's values and
+        // the second step's reads are live together and the compiler copies all eight).
+        for (; kf1 < kend; kf += 2.0f, kf1 += 2.0f) {
+          const float r0 = step(kf);
+          asm volatile("" ::"v"(r0) : "memory");
+          const float r1 = step(kf1);
+          acc += r0;
+          acc += r1;
+        }
+        if (kf < kend) acc += step(kf);
       } else {
         auto sample = [&](float k) {
           return lds_trilinear<DP, false>(base, PS, zbytes, fmaf(k, sd, pd0), fmaf(k, s1, p10),
