@@ -65,9 +65,10 @@ def test_project_f32_parity(gpu, h, mis):
                                              ("lds", None, "never"), ("lds", "600", "always"),
                                              ("lds", "600", "never"), ("lds", "0", None)])
 def test_project_layouts_agree(gpu, layout, cap, quad, monkeypatch):
-    # cap lowers the LDS brick budget so some (600) or all (0) slabs take the global fallback;
-    # quad forces every quad of views into shared (one brick per quarter tile for four views) or
-    # single (one view's whole tile per block) mode
+    # cap lowers the LDS brick budget so that slabs hold plain values instead of (value,
+    # difference) pairs and some (600) or all (0) slabs take the global fallback; quad forces
+    # every quad of views into shared (one brick per 16x8 half tile for four views) or single
+    # (one view's whole tile per block) mode
     from fastcone_amd import ops
     monkeypatch.setenv("FASTCONE_FP_LAYOUT", layout)
     if cap is not None:
