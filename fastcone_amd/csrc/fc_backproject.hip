@@ -177,6 +177,28 @@ __device__ __forceinline__ float tile_bilinear(const float* T, uint32_t iu, floa
   return lerp(lerp(p[0], p[1], au), lerp(p[kTileP], p[kTileP + 1], au), av);
 }
 
+// A zero dword that out-of-detector window pixels are copied from.
+__device__ float g_zero_word[1];
+
+// One dword per active lane from global memory into LDS at lds_dst + 4 lane (lds_dst
+// wave-uniform), by the load-to-LDS path: no registers hold the data and no LDS write
+// instruction is issued. Written as assembly rather than the compiler builtin, with which the
+// compiler waits for every outstanding load-to-LDS before each later LDS read (it cannot tell
+// that the reads address the other window buffer). The instruction has no register outputs, its
+// address registers are read at issue, and the LDS it writes is read only after the explicit
+// s_waitcnt and the barrier of the next view. Loads complete in order, so the compiler's own
+// vmcnt waits stay conservative with these loads outstanding.
+// M0 is declared clobbered; the compiler reports that it reserves M0, and it uses M0 nowhere
+// else in this kernel.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+__device__ __forceinline__ void copy_dword_to_lds(const float* src, float* lds_dst) {
+  const unsigned m0 = __builtin_amdgcn_readfirstlane((unsigned)(size_t)lds_dst);
+  asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dword %1, off" ::"s"(m0), "v"(src)
+               : "memory", "m0");
+}
+#pragma clang diagnostic pop
+
 template <int DT, int MODE>
 __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
   __shared__ float tile[2][kTileCap];
@@ -195,9 +217,9 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
   for (int k = 0; k < kZR; ++k) acc[k] = 0.0f;
 
   // Rows of a view's window, staged as (row wave + 4 q, column lane); zero outside the detector.
-  // The registers hold raw storage bits: converting at commit keeps the loads in flight. The
-  // row phase is wave-uniform, so row checks are scalar branches and addresses advance by a
-  // scalar stride.
+  // 16-bit storage: registers hold the raw storage bits (converting at commit keeps the loads in
+  // flight); FP32 storage copies straight into LDS (issue_lds below). The row phase is
+  // wave-uniform, so row checks are scalar branches and addresses advance by a scalar stride.
   const int wv = __builtin_amdgcn_readfirstlane(wave);
   uint32_t pre[kPre];
   auto issue = [&](int t, uint2 w) {
@@ -227,20 +249,52 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
     }
   };
 
+  // FP32 storage: the window rows go straight from global memory into the LDS buffer (rows of
+  // lanes < Wu; pixels outside the detector copy a zero), with no staging registers.
+  auto issue_lds = [&](int t, uint2 w, float* Tn) {
+    const uint32_t Wu = w.y & 0xffffu, Wv = w.y >> 16;
+    if (Wu == kWinSkip || Wu == kWinGlobal) return;
+    const int u0 = (int)(int16_t)(w.x & 0xffffu), v0 = (int)(int16_t)(w.x >> 16);
+    const int pu = u0 + lane;
+    const bool colok = pu >= 0 && pu < a.nw;
+    const int pv0 = v0 + wv;
+    const float* src0 = static_cast<const float*>(a.proj) + ((long long)t * a.nh + pv0) * a.nw + pu;
+    if ((uint32_t)lane < Wu) {
+#pragma unroll
+      for (int q = 0; q < kPre; ++q) {
+        const int pv = pv0 + 4 * q;
+        if ((uint32_t)(wv + 4 * q) < Wv) {
+          const bool ok = colok && pv >= 0 && pv < a.nh;
+          copy_dword_to_lds(ok ? src0 + (long long)(4 * q) * a.nw : g_zero_word,
+                            Tn + (uint32_t)(wv + 4 * q) * kTileP);
+        }
+      }
+    }
+  };
+
   uint2 wn = make_uint2(0u, kWinSkip);
   if (a.nviews > 0) {
     wn = W[0];
-    issue(0, wn);
+    if constexpr (DT == FC_F32) issue_lds(0, wn, tile[0]);
+    else issue(0, wn);
   }
   for (int t = 0; t < a.nviews; ++t) {
     const uint2 w = wn;
     float* T = tile[t & 1];
-    // the other buffer's readers (view t-1) all passed the previous barrier before this write
-    commit(T, w);
+    if constexpr (DT == FC_F32) {
+      // this wave's copies of view t have landed; the barrier then covers every wave's
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    } else {
+      // the other buffer's readers (view t-1) all passed the previous barrier before this write
+      commit(T, w);
+    }
     __syncthreads();
     if (t + 1 < a.nviews) {
       wn = W[t + 1];
-      issue(t + 1, wn);  // in flight while view t is accumulated
+      // in flight while view t is accumulated; the other buffer's readers (view t-1) all passed
+      // the barrier above
+      if constexpr (DT == FC_F32) issue_lds(t + 1, wn, tile[(t + 1) & 1]);
+      else issue(t + 1, wn);
     }
     const uint32_t Wu = w.y & 0xffffu, Wv = w.y >> 16;
     if (Wu == kWinSkip) continue;  // uniform: the cube misses the detector in this view
