@@ -120,6 +120,7 @@ struct FpArgs {
   int tiles_w, tiles_per_view, group;
   int max_steps;
   int lds_cap;  // LDS brick budget in floats (<= kLdsCap)
+  int pairs;    // 1: (value, difference) pair bricks where they fit; 0: plain values only
 };
 
 // Tile order: groups of `group` consecutive views; inside a group the same pixel tile of all
@@ -727,7 +728,7 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
         if (kf < kend) acc += sample(kf);
       }
     };
-    if (2 * NE <= lds_cap) slab(LayoutTag<true>{});
+    if (a.pairs && 2 * NE <= lds_cap) slab(LayoutTag<true>{});
     else slab(LayoutTag<false>{});
   }
   return acc;
@@ -1450,6 +1451,10 @@ int launch_project(int device, const void* vol, int dt, const int vs[3], const d
   // FASTCONE_FP_LDS_CAP lowers the LDS brick budget (testing the global-gather fallback)
   const char* cap = std::getenv("FASTCONE_FP_LDS_CAP");
   a.lds_cap = cap ? std::max(0, std::min(kLdsCap, std::atoi(cap))) : kLdsCap;
+  // FASTCONE_FP_PAIRS=0 keeps every LDS brick in plain values (testing that the pair bricks and
+  // their cell reuse give the same bits)
+  const char* pe = std::getenv("FASTCONE_FP_PAIRS");
+  a.pairs = pe && std::strcmp(pe, "0") == 0 ? 0 : 1;
   // The LDS projector deals quads of four views; `group` stays a multiple of four.
   const int nvq = (a.nviews + kQuad - 1) / kQuad * kQuad;
   const long long nblocks = (long long)a.tiles_per_view * (layout == FP_LDS ? nvq : a.nviews);

@@ -153,6 +153,31 @@ def test_project_16bit_storage_dense_views(gpu, dtype, quad, monkeypatch):
     assert rel(p, ref_rounded) < 5e-5
 
 
+@pytest.mark.parametrize("dtype", ["float32", "bfloat16"])
+@pytest.mark.parametrize("quad", ["always", "never"])
+def test_project_pair_bricks_bit_identical(gpu, dtype, quad, monkeypatch):
+    # A pair brick holds f(d) and f(d + 1) - f(d), the subtraction done once at the fill, and
+    # interpolates with fmaf(a, f1 - f0, f0): the operations lerp(f0, f1, a) performs on plain
+    # values, in the same order; the cell reuse only skips rereading the same four pairs. So the
+    # projections equal those of plain-value bricks (FASTCONE_FP_PAIRS=0) bit for bit, for x- and
+    # y-dominant views (angles near 10 and 100 degrees), shared quads and single views.
+    from fastcone_amd import ops
+    monkeypatch.setenv("FASTCONE_FP_LAYOUT", "lds")
+    monkeypatch.setenv("FASTCONE_FP_QUAD", quad)
+    dt = getattr(torch, dtype)
+    rng = np.random.default_rng(51)
+    ang = np.deg2rad(np.concatenate([10.0 + 0.5 * np.arange(8), 100.0 + 0.5 * np.arange(8)]))
+    g, _ = _setup(rng, ang=ang, vol=(20, 24, 22), vol_dim=(26, 28, 26), det=(40, 44),
+                  det_dim=(44, 48))
+    x = torch.from_numpy(rng.standard_normal(g.vol_shape).astype(np.float32)).to(gpu).to(dt)
+    p_pairs = ops.project(x, g, step=0.5)
+    monkeypatch.setenv("FASTCONE_FP_PAIRS", "0")
+    p_plain = ops.project(x, g, step=0.5)
+    torch.cuda.synchronize()
+    assert float(p_pairs.abs().max()) > 0.0
+    assert torch.equal(p_pairs, p_plain)
+
+
 @pytest.mark.parametrize("dtype", ["bfloat16", "float16"])
 def test_project_16bit_storage(gpu, dtype):
     from fastcone_amd import ops
