@@ -687,8 +687,8 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
       if constexpr (PAIR) {
         // A lane reads its cell's corner pairs only when the sample's base cell differs from its
         // previous sample's (a masked LDS read): samples 0.5 voxel apart along a ray share a cell
-        // about half the time, and the LDS is the bound. The first sample of a slab always reads
-        // (no element index is 2^32 - 1).
+        // about half the time, so about half of the LDS reads are skipped. The first sample of a
+        // slab always reads (no element index is 2^32 - 1).
         CellPairs C;
         C.e00 = C.e10 = C.e01 = C.e11 = make_float2(0.0f, 0.0f);
         uint32_t cur = 0xffffffffu;
