@@ -14,9 +14,10 @@
 // registers over every view ("wavefront-level accumulation over views"); a block owns a
 // 16 x 16 x 16 voxel cube. Per view the block stages, in LDS, the detector window that the cube
 // projects onto (rows of the raw projection, zero outside the detector), double-buffered: while
-// the block accumulates view t from one buffer, the rows of view t+1 are already in flight into
-// registers and are written to the other buffer at the top of the next iteration, so one barrier
-// per view suffices and the load latency hides behind the arithmetic. Per view and column the
+// the block accumulates view t from one buffer, the rows of view t+1 are already in flight (FP32
+// storage: copied straight into the other buffer; 16-bit storage: into registers, written to the
+// other buffer at the top of the next iteration), so one barrier per view suffices and the load
+// latency hides behind the arithmetic. Per view and column the
 // projective terms are evaluated once in FP64; when the detector columns do not depend on z
 // (untilted detectors) each voxel then costs one FMA for its row, a clamp and a bilinear
 // interpolation from LDS (two ds_read2 pairs).
@@ -199,6 +200,39 @@ __device__ __forceinline__ void copy_dword_to_lds(const float* src, float* lds_d
 }
 #pragma clang diagnostic pop
 
+// The per-view words every lane of a block needs (the view's 3 x 4 matrix, the next view's
+// window word and the view's weight), by scalar loads waited for in the same statement. With
+// the copies above in the loop the compiler cannot prove these words unchanged and reads them
+// with vector loads; vector loads count down in issue order, so waiting for one issued after
+// the next view's copies would also wait for every copy, and the view would not overlap them.
+typedef unsigned int fc_u32x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int fc_u32x16 __attribute__((ext_vector_type(16)));
+__device__ __forceinline__ void load_view_words(const double* M, const uint2* Wn, const float* S,
+                                                double m[12], uint2& wn, float& s) {
+  fc_u32x16 lo;
+  fc_u32x8 hi;
+  unsigned long long w;
+  unsigned int sv;
+  asm volatile(
+      "s_load_dwordx16 %0, %4, 0x0\n\t"
+      "s_load_dwordx8 %1, %4, 0x40\n\t"
+      "s_load_dwordx2 %2, %5, 0x0\n\t"
+      "s_load_dword %3, %6, 0x0\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&s"(lo), "=&s"(hi), "=&s"(w), "=&s"(sv)
+      : "s"(M), "s"(Wn), "s"(S)
+      : "memory");
+#pragma unroll
+  for (int i = 0; i < 8; ++i) m[i] = __hiloint2double((int)lo[2 * i + 1], (int)lo[2 * i]);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) m[8 + i] = __hiloint2double((int)hi[2 * i + 1], (int)hi[2 * i]);
+  wn = make_uint2((unsigned)w, (unsigned)(w >> 32));
+  s = __uint_as_float(sv);
+}
+
+// The weight read for views without a per-view norm.
+__device__ float g_one_word[1] = {1.0f};
+
 template <int DT, int MODE>
 __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
   __shared__ float tile[2][kTileCap];
@@ -289,20 +323,29 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
       commit(T, w);
     }
     __syncthreads();
-    if (t + 1 < a.nviews) {
-      wn = W[t + 1];
+    const bool more = t + 1 < a.nviews;
+    const uint32_t Wu = w.y & 0xffffu, Wv = w.y >> 16;
+    const double* Mp = a.views + (size_t)t * FC_VIEW_DOUBLES + 12;
+    double M[12];
+    float s;
+    if constexpr (DT == FC_F32) {
+      load_view_words(Mp, W + (more ? t + 1 : t), a.norm ? a.norm + t : g_one_word, M, wn, s);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 12; ++i) M[i] = Mp[i];
+      if (more) wn = W[t + 1];
+      s = a.norm ? a.norm[t] : 1.0f;
+    }
+    if (more) {
       // in flight while view t is accumulated; the other buffer's readers (view t-1) all passed
       // the barrier above
       if constexpr (DT == FC_F32) issue_lds(t + 1, wn, tile[(t + 1) & 1]);
       else issue(t + 1, wn);
     }
-    const uint32_t Wu = w.y & 0xffffu, Wv = w.y >> 16;
     if (Wu == kWinSkip) continue;  // uniform: the cube misses the detector in this view
-    const double* M = a.views + (size_t)t * FC_VIEW_DOUBLES + 12;
     const double A = M[0] * X + M[1] * Y + M[2] * Z0 + M[3];
     const double B = M[4] * X + M[5] * Y + M[6] * Z0 + M[7];
     const double C = M[8] * X + M[9] * Y + M[10] * Z0 + M[11];
-    const float s = a.norm ? a.norm[t] : 1.0f;
     if (Wu == kWinGlobal) {
       const size_t vbase = (size_t)t * a.nh * a.nw;
       const float af = (float)A, bf = (float)B, cf = (float)C;
