@@ -230,9 +230,6 @@ __device__ __forceinline__ void load_view_words(const double* M, const uint2* Wn
   s = __uint_as_float(sv);
 }
 
-// The weight read for views without a per-view norm.
-__device__ float g_one_word[1] = {1.0f};
-
 template <int DT, int MODE>
 __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
   __shared__ float tile[2][kTileCap];
@@ -264,11 +261,13 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
     const bool colok = (uint32_t)lane < Wu && pu >= 0 && pu < a.nw;
     const int pv0 = v0 + wv;
     const long long row0 = ((long long)t * a.nh + pv0) * a.nw;  // element index of (pv0, 0)
+    // row wv + 4 q is in the window when 4 q < rows: compared with an immediate, so no register
+    // holds a per-row constant across the view loop
+    const int rows = (int)Wv - wv;
 #pragma unroll
     for (int q = 0; q < kPre; ++q) {
       pre[q] = 0u;  // +0 in every storage format
-      const int pv = pv0 + 4 * q;
-      if ((uint32_t)(wv + 4 * q) < Wv && pv >= 0 && pv < a.nh && colok)
+      if (4 * q < rows && (uint32_t)(pv0 + 4 * q) < (uint32_t)a.nh && colok)
         pre[q] = load_bits<DT>(a.proj, (size_t)(row0 + (long long)(4 * q) * a.nw + pu));
     }
   };
@@ -276,10 +275,10 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
     const uint32_t Wu = w.y & 0xffffu, Wv = w.y >> 16;
     if (Wu == kWinSkip || Wu == kWinGlobal) return;
     float* Tw = T + (uint32_t)wv * kTileP + (uint32_t)lane;
+    const int rows = (int)Wv - wv;
 #pragma unroll
     for (int q = 0; q < kPre; ++q) {
-      if ((uint32_t)(wv + 4 * q) < Wv && (uint32_t)lane < Wu)
-        Tw[4u * kTileP * (uint32_t)q] = bits_to_f32<DT>(pre[q]);
+      if (4 * q < rows && (uint32_t)lane < Wu) Tw[4u * kTileP * (uint32_t)q] = bits_to_f32<DT>(pre[q]);
     }
   };
 
@@ -293,14 +292,17 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
     const bool colok = pu >= 0 && pu < a.nw;
     const int pv0 = v0 + wv;
     const float* src0 = static_cast<const float*>(a.proj) + ((long long)t * a.nh + pv0) * a.nw + pu;
+    // row wv + 4 q is in the window when 4 q < rows: compared with an immediate, so no register
+    // holds a per-row constant across the view loop
+    const int rows = (int)Wv - wv;
+    float* Tw = Tn + (uint32_t)wv * kTileP;
     if ((uint32_t)lane < Wu) {
 #pragma unroll
       for (int q = 0; q < kPre; ++q) {
-        const int pv = pv0 + 4 * q;
-        if ((uint32_t)(wv + 4 * q) < Wv) {
-          const bool ok = colok && pv >= 0 && pv < a.nh;
+        if (4 * q < rows) {
+          const bool ok = colok && (uint32_t)(pv0 + 4 * q) < (uint32_t)a.nh;
           copy_dword_to_lds(ok ? src0 + (long long)(4 * q) * a.nw : g_zero_word,
-                            Tn + (uint32_t)(wv + 4 * q) * kTileP);
+                            Tw + 4u * kTileP * (uint32_t)q);
         }
       }
     }
@@ -328,19 +330,22 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
     const double* Mp = a.views + (size_t)t * FC_VIEW_DOUBLES + 12;
     double M[12];
     float s;
+    // the next view's rows are in flight while view t is accumulated; the other buffer's readers
+    // (view t-1) all passed the barrier above
     if constexpr (DT == FC_F32) {
-      load_view_words(Mp, W + (more ? t + 1 : t), a.norm ? a.norm + t : g_one_word, M, wn, s);
+      // without norms the weight's load reads the matrix's first word, and the weight is 1
+      load_view_words(Mp, W + (more ? t + 1 : t),
+                      a.norm ? a.norm + t : reinterpret_cast<const float*>(Mp), M, wn, s);
+      if (!a.norm) s = 1.0f;
+      if (more) issue_lds(t + 1, wn, tile[(t + 1) & 1]);
     } else {
+      if (more) {
+        wn = W[t + 1];
+        issue(t + 1, wn);
+      }
 #pragma unroll
       for (int i = 0; i < 12; ++i) M[i] = Mp[i];
-      if (more) wn = W[t + 1];
       s = a.norm ? a.norm[t] : 1.0f;
-    }
-    if (more) {
-      // in flight while view t is accumulated; the other buffer's readers (view t-1) all passed
-      // the barrier above
-      if constexpr (DT == FC_F32) issue_lds(t + 1, wn, tile[(t + 1) & 1]);
-      else issue(t + 1, wn);
     }
     if (Wu == kWinSkip) continue;  // uniform: the cube misses the detector in this view
     const double A = M[0] * X + M[1] * Y + M[2] * Z0 + M[3];
