@@ -658,14 +658,18 @@ __device__ __forceinline__ float march(const FpArgs& a, float* brick, int4* tab,
           const uint32_t m2 = (uint32_t)(((float)c + 0.5f) * invE1);  // exact: c < 2^10
           const uint32_t m1 = (uint32_t)c - mul_u24_s(m2, (uint32_t)E1);
           const uint32_t g = gbase + mad_u24_s(m1, X.gs1, mul_u24_s(m2, X.gs2));
-          // the column's address once; each element adds a wave-uniform offset
+          // the column's address once; each element adds a wave-uniform offset, recomputed by
+          // the scalar unit per column (the empty asm redefines its inputs), so that the DP
+          // offsets do not hold scalar registers across the fill loop
           const void* colp =
               static_cast<const char*>(a.data) + (size_t)g * (DT == FC_F32 ? 4u : 2u);
+          uint32_t gsd = X.gsd, dm = dmax;
+          asm volatile("" : "+s"(gsd), "+s"(dm));
           float v[DP];
 #pragma unroll
           for (int d = 0; d < DP; ++d)
             v[d] = load_elem<DT>(colp, (size_t)(uint32_t)__builtin_amdgcn_readfirstlane(
-                                           min((uint32_t)d, dmax) * X.gsd));
+                                           min((uint32_t)d, dm) * gsd));
           const uint32_t col = mad_u24_s(m2, pad, mul_u24((uint32_t)c, (uint32_t)DP));
 #pragma unroll
           for (int d = 0; d < DP; ++d)
