@@ -178,9 +178,6 @@ __device__ __forceinline__ float tile_bilinear(const float* T, uint32_t iu, floa
   return lerp(lerp(p[0], p[1], au), lerp(p[kTileP], p[kTileP + 1], au), av);
 }
 
-// A zero dword that out-of-detector window pixels are copied from.
-__device__ float g_zero_word[1];
-
 // One dword per active lane from global memory into LDS at lds_dst + 4 lane (lds_dst
 // wave-uniform), by the load-to-LDS path: no registers hold the data and no LDS write
 // instruction is issued. Written as assembly rather than the compiler builtin, with which the
@@ -283,7 +280,7 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
   };
 
   // FP32 storage: the window rows go straight from global memory into the LDS buffer (rows of
-  // lanes < Wu; pixels outside the detector copy a zero), with no staging registers.
+  // lanes < Wu; pixels outside the detector are stored as zero), with no staging registers.
   auto issue_lds = [&](int t, uint2 w, float* Tn) {
     const uint32_t Wu = w.y & 0xffffu, Wv = w.y >> 16;
     if (Wu == kWinSkip || Wu == kWinGlobal) return;
@@ -300,9 +297,12 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
 #pragma unroll
       for (int q = 0; q < kPre; ++q) {
         if (4 * q < rows) {
-          const bool ok = colok && (uint32_t)(pv0 + 4 * q) < (uint32_t)a.nh;
-          copy_dword_to_lds(ok ? src0 + (long long)(4 * q) * a.nw : g_zero_word,
-                            Tw + 4u * kTileP * (uint32_t)q);
+          // pixels outside the detector are written as zero by the lanes themselves
+          float* dst = Tw + 4u * kTileP * (uint32_t)q;
+          if (colok && (uint32_t)(pv0 + 4 * q) < (uint32_t)a.nh)
+            copy_dword_to_lds(src0 + (long long)(4 * q) * a.nw, dst);
+          else
+            dst[lane] = 0.0f;
         }
       }
     }
@@ -317,7 +317,18 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
   for (int t = 0; t < a.nviews; ++t) {
     const uint2 w = wn;
     float* T = tile[t & 1];
+    const bool more = t + 1 < a.nviews;
+    const uint32_t Wu = w.y & 0xffffu, Wv = w.y >> 16;
+    const double* Mp = a.views + (size_t)t * FC_VIEW_DOUBLES + 12;
+    double M[12];
+    float s;
     if constexpr (DT == FC_F32) {
+      // the view's words first (they do not depend on the barrier: a wave that finishes view
+      // t - 1 early waits for them while the others still compute); without norms the weight's
+      // load reads the matrix's first word, and the weight is 1
+      load_view_words(Mp, W + (more ? t + 1 : t),
+                      a.norm ? a.norm + t : reinterpret_cast<const float*>(Mp), M, wn, s);
+      if (!a.norm) s = 1.0f;
       // this wave's copies of view t have landed; the barrier then covers every wave's
       asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     } else {
@@ -325,18 +336,9 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
       commit(T, w);
     }
     __syncthreads();
-    const bool more = t + 1 < a.nviews;
-    const uint32_t Wu = w.y & 0xffffu, Wv = w.y >> 16;
-    const double* Mp = a.views + (size_t)t * FC_VIEW_DOUBLES + 12;
-    double M[12];
-    float s;
     // the next view's rows are in flight while view t is accumulated; the other buffer's readers
     // (view t-1) all passed the barrier above
     if constexpr (DT == FC_F32) {
-      // without norms the weight's load reads the matrix's first word, and the weight is 1
-      load_view_words(Mp, W + (more ? t + 1 : t),
-                      a.norm ? a.norm + t : reinterpret_cast<const float*>(Mp), M, wn, s);
-      if (!a.norm) s = 1.0f;
       if (more) issue_lds(t + 1, wn, tile[(t + 1) & 1]);
     } else {
       if (more) {
