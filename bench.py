@@ -160,7 +160,7 @@ def main():
                     "count of the clipped rays",
         "units_per_launch": samples, "taps_per_s": fp_rate,
         "peak_kind": "LDS bandwidth: 128 B/clock/CU x 256 CUs x 2.4 GHz (published figures)",
-        "limiter": "VALU issue (1.29 wave instructions per CU clock by the SQ counters in "
+        "limiter": "VALU issue (1.25-1.30 wave instructions per CU clock by the SQ counters in "
                    "profiles/fp_quad_kernel_pmc.md); a lane rereads its cell's corner pairs only "
                    "when the cell changes, so the LDS moves about 60 % of the algorithmic bytes",
         "ceilings": {
