@@ -178,9 +178,10 @@ __device__ __forceinline__ float tile_bilinear(const float* T, uint32_t iu, floa
   return lerp(lerp(p[0], p[1], au), lerp(p[kTileP], p[kTileP + 1], au), av);
 }
 
-// One dword per active lane from global memory into LDS at lds_dst + 4 lane (lds_dst
-// wave-uniform), by the load-to-LDS path: no registers hold the data and no LDS write
-// instruction is issued. Written as assembly rather than the compiler builtin, with which the
+// One dword per active lane from global memory at row + lane_bytes (row wave-uniform, in scalar
+// registers: the scalar-base form of the instruction, so a row costs no vector arithmetic) into
+// LDS at lds_dst + 4 lane (lds_dst wave-uniform), by the load-to-LDS path: no registers hold the
+// data and no LDS write instruction is issued. Written as assembly rather than the compiler builtin, with which the
 // compiler waits for every outstanding load-to-LDS before each later LDS read (it cannot tell
 // that the reads address the other window buffer). The instruction has no register outputs, its
 // address registers are read at issue, and the LDS it writes is read only after the explicit
@@ -190,9 +191,11 @@ __device__ __forceinline__ float tile_bilinear(const float* T, uint32_t iu, floa
 // else in this kernel.
 #pragma clang diagnostic push
 #pragma clang diagnostic ignored "-Winline-asm"
-__device__ __forceinline__ void copy_dword_to_lds(const float* src, float* lds_dst) {
+__device__ __forceinline__ void copy_dword_to_lds(const float* row, uint32_t lane_bytes,
+                                                  float* lds_dst) {
   const unsigned m0 = __builtin_amdgcn_readfirstlane((unsigned)(size_t)lds_dst);
-  asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dword %1, off" ::"s"(m0), "v"(src)
+  asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dword %1, %2" ::"s"(m0), "v"(lane_bytes),
+               "s"(row)
                : "memory", "m0");
 }
 #pragma clang diagnostic pop
@@ -288,7 +291,10 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
     const int pu = u0 + lane;
     const bool colok = pu >= 0 && pu < a.nw;
     const int pv0 = v0 + wv;
-    const float* src0 = static_cast<const float*>(a.proj) + ((long long)t * a.nh + pv0) * a.nw + pu;
+    // the window row's first pixel (wave-uniform; it may lie before the detector row, but only
+    // lanes with a pixel on the detector load) and each lane's byte offset from it
+    const float* row0 = static_cast<const float*>(a.proj) + ((long long)t * a.nh + pv0) * a.nw + u0;
+    const uint32_t lane_bytes = 4u * (uint32_t)lane;
     // row wv + 4 q is in the window when 4 q < rows: compared with an immediate, so no register
     // holds a per-row constant across the view loop
     const int rows = (int)Wv - wv;
@@ -300,7 +306,7 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
           // pixels outside the detector are written as zero by the lanes themselves
           float* dst = Tw + 4u * kTileP * (uint32_t)q;
           if (colok && (uint32_t)(pv0 + 4 * q) < (uint32_t)a.nh)
-            copy_dword_to_lds(src0 + (long long)(4 * q) * a.nw, dst);
+            copy_dword_to_lds(row0 + (long long)(4 * q) * a.nw, lane_bytes, dst);
           else
             dst[lane] = 0.0f;
         }
