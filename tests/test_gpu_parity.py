@@ -242,6 +242,24 @@ def test_backproject_accumulates_into_out(gpu):
     assert rel(out.cpu().numpy() - base, ref) < 5e-5
 
 
+@pytest.mark.parametrize("nviews", [1, 2])
+@pytest.mark.parametrize("dtype", ["float32", "bfloat16"])
+def test_backproject_one_and_two_views(gpu, nviews, dtype):
+    # the view loop's first and last views: the next view's window word and rows are read only
+    # while a next view exists
+    from fastcone_amd import ops
+    dt = getattr(torch, dtype)
+    rng = np.random.default_rng(27 + nviews)
+    g, prob = _setup(rng, ang=np.sort(rng.uniform(0, 2 * np.pi, nviews)))
+    y = rng.standard_normal(g.proj_shape).astype(np.float32)
+    norm = rng.uniform(0.5, 1.5, nviews).astype(np.float32)
+    v = ops.backproject(torch.from_numpy(y).to(gpu).to(dt), g, mode=2,
+                        norm=torch.from_numpy(norm)).cpu().numpy()
+    ref = oracle.backproject(prob, _round(y, dt), g.vol_shape, mode=2, norm=norm)
+    assert float(np.abs(ref).max()) > 0.0
+    assert rel(v, ref) < 5e-5
+
+
 @pytest.mark.parametrize("layout", ["lds", "plain"])
 def test_adjoint_paths_agree(gpu, layout, monkeypatch):
     # lds: voxel-owned bricks with LDS accumulation (default); plain: global-atomic scatter
