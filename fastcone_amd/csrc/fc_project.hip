@@ -1031,6 +1031,8 @@ __global__ __launch_bounds__(256) void adj_windows(AdjVArgs a, uint2* __restrict
   out[idx] = w;
 }
 
+// CAPRUN: cap a run at kRun samples (steps below 0.25 voxel; see the run merging below).
+template <bool CAPRUN>
 __global__ __launch_bounds__(kAT) void adjoint_vox_kernel(AdjVArgs a) {
   __shared__ unsigned long long acc[kAL3];
   for (int e = threadIdx.x; e < kAL3; e += kAT) acc[e] = 0ull;
@@ -1117,8 +1119,10 @@ __global__ __launch_bounds__(kAT) void adjoint_vox_kernel(AdjVArgs a) {
     if (!(khi >= klo)) continue;
     const int k0 = max(0, (int)ceilf(klo)), k1 = min(r.nk - 1, (int)floorf(khi));
     // Consecutive samples (0.5 voxel apart) often share a base cell: their weights are summed in
-    // registers and added to LDS when the cell changes or after kRun samples (so that a sum
-    // stays below 2^31 in the 2^28 fixed-point scale).
+    // registers and added to LDS when the cell changes. A sum must stay below 2^31 in the 2^28
+    // fixed-point scale, so at most 7 samples. The step h along a unit direction advances the
+    // dominant coordinate by at least h / sqrt(3), so a cell holds at most floor(sqrt(3) / h) + 1
+    // samples: 7 for h >= 0.25, and the run needs no count. Shorter steps flush after kRun.
     constexpr int kRun = 4;
     float A[8];
 #pragma unroll
@@ -1148,14 +1152,14 @@ __global__ __launch_bounds__(kAT) void adjoint_vox_kernel(AdjVArgs a) {
       const float w10 = wz0 * ay, w00 = wz0 - w10, w11 = wz1 * ay, w01 = wz1 - w11;
       const float w[8] = {w00 - w00 * ax, w00 * ax, w10 - w10 * ax, w10 * ax,
                           w01 - w01 * ax, w01 * ax, w11 - w11 * ax, w11 * ax};
-      if (cell != cur || run == kRun) {
+      if (cell != cur || (CAPRUN && run == kRun)) {
         if (cur != 0xffffffffu) flush();
         cur = cell;
-        run = 0;
+        if constexpr (CAPRUN) run = 0;
 #pragma unroll
         for (int c = 0; c < 8; ++c) A[c] = 0.0f;
       }
-      ++run;
+      if constexpr (CAPRUN) ++run;
 #pragma unroll
       for (int c = 0; c < 8; ++c) A[c] += w[c];
     }
@@ -1581,7 +1585,8 @@ int launch_adjoint(int device, float* vol, const int vs[3], const double origin[
   else adj_rays<FC_F16><<<rblocks, 256, 0, stream>>>(a);
   adj_windows<<<(unsigned)((nwin + 255) / 256), 256, 0, stream>>>(a, win);
   const dim3 grid(a.nbx, a.nby, a.nbz * a.nsplit);
-  adjoint_vox_kernel<<<grid, kAT, 0, stream>>>(a);
+  if (a.h >= 0.25) adjoint_vox_kernel<false><<<grid, kAT, 0, stream>>>(a);
+  else adjoint_vox_kernel<true><<<grid, kAT, 0, stream>>>(a);
   prof_end(tok, stream);
   FC_HIP(hipGetLastError());
   return release_ws(device, SLOT_OCTET, stream);

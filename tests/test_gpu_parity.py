@@ -283,7 +283,8 @@ def test_adjoint_steep_and_wide_cone(gpu):
     assert rel(aty, ref) < 5e-5
 
 
-@pytest.mark.parametrize("h", [1.0, 0.5])
+# 0.3: runs of up to 6 same-cell samples merged without a count; 0.2: runs capped at 4
+@pytest.mark.parametrize("h", [1.0, 0.5, 0.3, 0.2])
 def test_adjoint_parity_and_dot_product(gpu, h):
     from fastcone_amd import ops
     rng = np.random.default_rng(30)
