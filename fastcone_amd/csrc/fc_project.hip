@@ -1139,8 +1139,10 @@ __global__ __launch_bounds__(kAT) void adjoint_vox_kernel(AdjVArgs a) {
       add(L + kAL * kAL, A[4]); add(L + kAL * kAL + 1, A[5]);
       add(L + kAL * kAL + kAL, A[6]); add(L + kAL * kAL + kAL + 1, A[7]);
     };
-    for (int k = k0; k <= k1; ++k) {
-      const float kf = (float)k;
+    // the sample index counts in FP32 (exact: k < 2^24), the same values as (float)k, so one add
+    // and one compare per sample control the loop
+    const float kf1 = (float)k1;
+    for (float kf = (float)k0; kf <= kf1; kf += 1.0f) {
       const float qx = fmaf(kf, r.sx, r.px), qy = fmaf(kf, r.sy, r.py), qz = fmaf(kf, r.sz, r.pz);
       const uint32_t cx = (uint32_t)((int)qx - lox), cy = (uint32_t)((int)qy - loy),
                      cz = (uint32_t)((int)qz - loz);
@@ -1419,6 +1421,8 @@ int launch_project(int device, const void* vol, int dt, const int vs[3], const d
   }
   FC_CHECK_ARG(PY <= 65535 && PZ <= 65535, "volume dimension too large for staging grid");
   FC_CHECK_ARG((size_t)PX * PY * PZ < (1ull << 32), "volume too large for 32-bit voxel offsets");
+  FC_CHECK_ARG(max_steps_for(vs, h) <= (1 << 24), "step %g too small for the volume: over 2^24 "
+               "samples per ray (the sample loops count in FP32)", h);
   // one mode byte per quad of the LDS projector after the staged volume
   const size_t flags_off = (bytes + 255) & ~(size_t)255;
   const size_t nquads = (size_t)((ps[0] + kTile - 1) / kTile) * ((ps[1] + kTile - 1) / kTile) *
@@ -1511,6 +1515,8 @@ int launch_adjoint(int device, float* vol, const int vs[3], const double origin[
   FC_CHECK_ARG((size_t)PX * PY * PZ < (1ull << 32), "volume too large for 32-bit voxel offsets");
   FC_CHECK_ARG(ps[0] <= 65535 && ps[1] <= 65535, "detector too large for the adjoint windows");
   if (ps[2] == 0 || nx == 0 || ny == 0 || nz == 0) return FC_OK;
+  FC_CHECK_ARG(max_steps_for(vs, h) <= (1 << 24), "step %g too small for the volume: over 2^24 "
+               "samples per ray (the sample loops count in FP32)", h);
   if (fp_layout_choice() != FP_LDS) {
     // global-atomic scatter into a padded accumulator (reference path for tests)
     size_t n = (size_t)PX * PY * PZ;
@@ -1681,6 +1687,8 @@ int launch_sample_count(int device, const int vs[3], const double origin[3], con
   a.tiles_w = (a.nw + kTile - 1) / kTile;
   a.tiles_per_view = a.tiles_w * ((a.nh + kTile - 1) / kTile);
   a.max_steps = max_steps_for(vs, h);
+  FC_CHECK_ARG(a.max_steps <= (1 << 24), "step %g too small for the volume: over 2^24 "
+               "samples per ray (the sample loops count in FP32)", h);
   FC_CHECK_ARG(a.nviews <= 65535, "too many views for the count grid");
   if (a.nviews > 0) {
     zero_fill<long long><<<(a.nviews + 255) / 256, 256, 0, stream>>>(counts, (size_t)a.nviews);

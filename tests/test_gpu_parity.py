@@ -301,6 +301,25 @@ def test_adjoint_parity_and_dot_product(gpu, h):
     assert abs(lhs - rhs) / abs(lhs) < 1e-5
 
 
+def test_step_too_small_for_the_volume_is_rejected(gpu):
+    # the sample loops count in FP32, exact below 2^24: a 16384 x 3700 slice (the largest
+    # dimension allowed, diagonal 16.8k voxels) at step 1e-3 could need 1.68e7 samples per ray,
+    # so the projector and the adjoint refuse it instead of rounding k
+    from fastcone_amd import ops
+    from fastcone_amd._native import FastconeError
+    from fastcone_amd.geometry import Geometry
+    g = Geometry.circular(np.array([0.3]), 60000.0, 1.0, (1, 3700, 16384), (8, 8))
+    x = torch.zeros(g.vol_shape, device=gpu)
+    with pytest.raises(FastconeError, match="2\\^24"):
+        ops.project(x, g, step=1e-3)
+    y = torch.zeros(g.proj_shape, device=gpu)
+    with pytest.raises(FastconeError, match="2\\^24"):
+        ops.adjoint(y, g, step=1e-3)
+    # the same geometry at step 0.5 runs
+    ops.project(x, g, step=0.5)
+    torch.cuda.synchronize()
+
+
 def test_sample_count_exact(gpu):
     from fastcone_amd import ops
     rng = np.random.default_rng(31)
