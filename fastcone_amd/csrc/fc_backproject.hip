@@ -181,12 +181,12 @@ __device__ __forceinline__ float tile_bilinear(const float* T, uint32_t iu, floa
 // One dword per active lane from global memory at row + lane_bytes (row wave-uniform, in scalar
 // registers: the scalar-base form of the instruction, so a row costs no vector arithmetic) into
 // LDS at lds_dst + 4 lane (lds_dst wave-uniform), by the load-to-LDS path: no registers hold the
-// data and no LDS write instruction is issued. Written as assembly rather than the compiler builtin, with which the
-// compiler waits for every outstanding load-to-LDS before each later LDS read (it cannot tell
-// that the reads address the other window buffer). The instruction has no register outputs, its
-// address registers are read at issue, and the LDS it writes is read only after the explicit
-// s_waitcnt and the barrier of the next view. Loads complete in order, so the compiler's own
-// vmcnt waits stay conservative with these loads outstanding.
+// data and no LDS write instruction is issued. Written as assembly rather than the compiler
+// builtin, with which the compiler waits for every outstanding load-to-LDS before each later LDS
+// read (it cannot tell that the reads address the other window buffer). The instruction has no
+// register outputs, its address registers are read at issue, and the LDS it writes is read only
+// after the explicit s_waitcnt and the barrier of the next view. Loads complete in order, so the
+// compiler's own vmcnt waits stay conservative with these loads outstanding.
 // M0 is declared clobbered; the compiler reports that it reserves M0, and it uses M0 nowhere
 // else in this kernel.
 #pragma clang diagnostic push
@@ -278,7 +278,8 @@ __global__ __launch_bounds__(256) void bp_kernel(BpArgs a) {
     const int rows = (int)Wv - wv;
 #pragma unroll
     for (int q = 0; q < kPre; ++q) {
-      if (4 * q < rows && (uint32_t)lane < Wu) Tw[4u * kTileP * (uint32_t)q] = bits_to_f32<DT>(pre[q]);
+      if (4 * q < rows && (uint32_t)lane < Wu)
+        Tw[4u * kTileP * (uint32_t)q] = bits_to_f32<DT>(pre[q]);
     }
   };
 
